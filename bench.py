@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the FLIP/PIC step on one MI355X.
+
+Workload (default): the 128^3 high-density tank - lower 40% filled at 8
+particles per cell (6,684,672 particles, seed 1), velocity U[-0.5,0.5]^3,
+dt 1/240, FLIP 0.95, CG to a 1e-5 relative residual (cap 200), warm start,
+sort every substep.  Inputs are synthetic and resident in HBM before timing.
+
+value     = particle-steps/s of the mixed-precision fused step (f32 vectors,
+            f64 CG dots and true residual): N * K / seconds for K steps,
+            device-synchronised on both sides.
+plain     = the same for the reference-structured FP32 step (one kernel per
+            WGSL pass, f32 reduction trees, host-driven CG loop).
+roofline  = the persistent CG kernel: algorithmic bytes per launch over its
+            event-timed average duration, against 8 TB/s.
+cpu_baseline = the CPU restatement (oracle/, plain FP32, OpenMP on all host
+            cores) timed on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("fluid-simulation_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+
+
+def cg_bytes(cells, iters):
+    """Compulsory CG traffic for one solve over `cells` cells and `iters`
+    iterations: 45 B/cell/iteration (SpMV+d.q: read d 4 + flag 1, write q 4;
+    update+r.r: read p,d,r,q 16, write p,r 8; direction: read r,d 8, write d
+    4) plus 30 B/cell per solve (init 21 + f64 true residual 9)."""
+    return 45.0 * cells * iters + 30.0 * cells
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "cg_persistent_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def run_gpu(fsm, sc, params, steps, warmup, timing_steps=3):
+    pos, vel = sc["pos"], sc["vel"]
+    t = fsm.FluidStep(params, pos, vel)
+    t.set_timing(False)
+    if warmup:
+        t.step(warmup)
+    t.synchronize()
+    t0 = time.perf_counter()
+    t.step(steps)
+    t.synchronize()
+    dt = time.perf_counter() - t0
+    # Per-launch CG timing with device events (same stream as the kernels).
+    t.set_timing(True)
+    per = []
+    for _ in range(timing_steps):
+        t.step(1)
+        per.append(t.stats())
+    t.close()
+    return dt, per
+
+
+def cpu_baseline(sc, steps):
+    import oracle
+
+    import helpers  # tests/helpers.py
+
+    P = helpers.oracle_params(sc, 0)
+    t0 = time.perf_counter()
+    res = oracle.run(P, sc["pos"], sc["vel"], steps=steps, sort_period=1, want=False)
+    dt = time.perf_counter() - t0
+    return dt, res, oracle.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--bf16", action="store_true", help="bf16 particle-velocity storage")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} (WORLD_SIZE={world}) is not supported; "
+              "the step runs on a single MI355X and multi-GPU sharding is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import fluidstep as fsm
+    import scenes
+    from helpers import gpu_params
+
+    sc = scenes.high_density_128()
+    N = len(sc["pos"])
+    vstore = fsm.VELOCITY_BF16 if args.bf16 else fsm.VELOCITY_F32
+    fused = gpu_params(sc, fsm.ENGINE_FUSED, velocity=vstore)
+    dt, per = run_gpu(fsm, sc, fused, args.steps, args.warmup)
+    value = N * args.steps / dt
+    ms_per_step = dt / args.steps * 1e3
+
+    # Roofline of the persistent CG kernel (dominant kernel of the step).
+    ach, ach_full, frac, cg_ms, iters, act = [], [], [], [], [], []
+    C = sc["n"] ** 3
+    for s in per:
+        k = s["cg_iterations"]
+        Ca = s["cg_active_cells"]
+        sec = s["ms_cg"] / 1e3
+        if sec > 0:
+            ach.append(cg_bytes(Ca, k) / sec / 1e9)
+            ach_full.append(cg_bytes(C, k) / sec / 1e9)
+        cg_ms.append(s["ms_cg"])
+        iters.append(k)
+        act.append(Ca)
+    achieved = float(np.mean(ach)) if ach else None
+    traffic = load_traffic()
+
+    line = {
+        "metric": "FLIP particle-steps/s @128^3: mixed vs FP32 CG at 1e-5 residual; CG %HBM peak",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "precision": "mixed: f32 vectors and particles%s, f64 CG dots/scalars/true residual"
+                     % (" (bf16 particle velocity)" if args.bf16 else ""),
+        "data": "synthetic",
+        "config": {
+            "workload": "high_density_128", "grid": [128, 128, 128], "particles": N, "dt": 1 / 240,
+            "flip": 0.95, "cg_tolerance": 1e-5, "cg_cap": 200, "sort_period": 1, "engine": "fused",
+        },
+        "cg": {"iterations_per_solve": iters, "active_cells": act, "ms_per_solve": cg_ms,
+               "ms_per_iteration": [m / max(k, 1) for m, k in zip(cg_ms, iters)]},
+        "sections_ms": {k: per[-1][k] for k in ("ms_prep", "ms_pressure", "ms_finish", "ms_cg")},
+        "roofline": {
+            "kernel": "k_cg_persistent",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
+            "basis": "45 B/cell/iteration + 30 B/cell/solve over the cells of liquid tiles",
+            "achieved_full_grid_basis": float(np.mean(ach_full)) if ach_full else None,
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        },
+    }
+
+    if not args.no_plain:
+        plain = gpu_params(sc, fsm.ENGINE_REFERENCE)
+        pdt, pper = run_gpu(fsm, sc, plain, args.plain_steps, 1, timing_steps=1)
+        pv = N * args.plain_steps / pdt
+        line["plain_fp32"] = {"value": pv, "unit": "particle-steps/s", "ms_per_step": pdt / args.plain_steps * 1e3,
+                              "cg_iterations": pper[-1]["cg_iterations"], "ms_cg": pper[-1]["ms_cg"]}
+        line["speedup_vs_plain_fp32"] = value / pv
+
+    if not args.no_cpu:
+        cdt, res, cores = cpu_baseline(sc, args.cpu_steps)
+        line["cpu_baseline"] = {
+            "value": N * args.cpu_steps / cdt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{args.cpu_steps} substeps of the 128^3 workload from its initial state, plain FP32 "
+                      f"restatement (oracle/), OpenMP {cores} threads, {cdt:.1f} s",
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

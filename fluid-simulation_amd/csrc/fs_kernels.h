@@ -1,0 +1,128 @@
+// fs_kernels.h — host-side launchers of the step's HIP kernels.  Every
+// launcher enqueues on the given stream and returns the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fs_device.h"
+
+namespace fs {
+
+// Particle SoA on one ping-pong side.  Velocity is f32 or bf16 storage.
+struct ParticleSoA {
+    float* pos[3];
+    void* vel[3];
+};
+
+// ---- k_bin.hip: clears, keys, histogram, scans, stable radix permutation ----
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+hipError_t launch_fill_f32(float* p, float value, size_t count, hipStream_t s);
+// sh/mark.wgsl as written: one global atomic per particle.
+hipError_t launch_mark(const StepParams& P, const float* const pos[3], uint32_t* occ, hipStream_t s);
+// Keys + histogram with wave-aggregated atomics; keys[p] = cell_key(p).
+// misfit counts particles whose P2G cell (floor((pos-origin)*inv_h)) differs
+// from their sort cell; the brick gather needs it to be zero.
+hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys,
+                           uint32_t* occ, uint32_t* misfit, hipStream_t s);
+hipError_t launch_fill_u32_guarded(uint32_t* p, uint32_t value, size_t count, const uint32_t* guard,
+                                   hipStream_t s);
+// Exclusive scan workspace size (u32 elements) for `len` inputs.
+size_t scan_workspace(size_t len);
+hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
+                                 hipStream_t s);
+size_t radix_workspace(size_t n);  // u32 elements (histogram table + scan ws)
+// Stable LSD radix sort of keys[0..n) (keys < key_limit); writes perm[d] = the
+// source index of sorted slot d.  keys_tmp/vals_tmp are n-sized scratch.
+hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t key_limit,
+                                     uint32_t* keys_tmp0, uint32_t* keys_tmp1, uint32_t* vals_tmp,
+                                     uint32_t* perm, uint32_t* ws, hipStream_t s);
+// dst[d] = src[perm[d]] for the six particle columns.
+hipError_t launch_permute(uint32_t n, const uint32_t* perm, const ParticleSoA& src,
+                          const ParticleSoA& dst, int bf16, hipStream_t s);
+
+// ---- k_grid.hip: classify, P2G, face passes, divergence, gradient ----
+hipError_t launch_classify(const StepParams& P, const uint32_t* occ, uint8_t* ct, uint32_t* stats,
+                           hipStream_t s);
+// sh/scatter.wgsl: per-particle i32 atomics into num/den.
+// guard (nullable): run only when *guard != 0.
+hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf16,
+                             int32_t* const num[3], int32_t* const den[3], const uint32_t* guard,
+                             hipStream_t s);
+// Reference-structured per-axis passes (one kernel each, sh/*.wgsl).
+hipError_t launch_normalize(const StepParams& P, int a, const int32_t* num, const int32_t* den,
+                            float* vel, hipStream_t s);
+hipError_t launch_copy_f32(const float* src, float* dst, size_t n, hipStream_t s);
+hipError_t launch_forces(const StepParams& P, int a, const uint8_t* ct, float* vel, hipStream_t s);
+hipError_t launch_enforce(const StepParams& P, int a, const uint8_t* ct, float* vel, hipStream_t s);
+hipError_t launch_gradient(const StepParams& P, int a, const float* p, const uint8_t* ct, float* vel,
+                           hipStream_t s);
+hipError_t launch_divergence(const StepParams& P, float* const vel[3], const uint8_t* ct,
+                             const uint32_t* occ, float* div, hipStream_t s);
+// Fused: normalize -> save -> forces -> enforce for all three axes.
+hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3],
+                                const int32_t* const den[3], const uint8_t* ct, float* const vel[3],
+                                float* const saved[3], const uint32_t* guard, hipStream_t s);
+// Fused: gradient -> enforce for all three axes.
+hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const uint8_t* ct,
+                                   float* const vel[3], hipStream_t s);
+// Sorted-segment P2G gather per brick of cells, with the face epilogue fused.
+hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, int bf16,
+                             const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct,
+                             float* const vel[3], float* const saved[3], const uint32_t* misfit,
+                             hipStream_t s);
+
+// ---- k_cg.hip: pressure solve ----
+// Reference-structured CG passes (sh/cg_*.wgsl), f32 scalars in `scal`.
+hipError_t launch_cg_init_ref(const StepParams& P, const float* div, const uint8_t* ct, float* p,
+                              float* r, float* d, float* scal, hipStream_t s);
+hipError_t launch_cg_reduce_ref(const StepParams& P, const float* a, const float* b, float* partials,
+                                const float* scal, hipStream_t s);
+hipError_t launch_cg_reduce_final_ref(const StepParams& P, const float* partials, float* scal,
+                                      hipStream_t s);
+hipError_t launch_cg_set_rsold_ref(float* scal, hipStream_t s);
+hipError_t launch_cg_spmv_ref(const StepParams& P, const uint8_t* ct, const float* d, float* q,
+                              const float* scal, hipStream_t s);
+hipError_t launch_cg_alpha_ref(float* scal, hipStream_t s);
+hipError_t launch_cg_update_ref(const StepParams& P, const float* scal, const float* d, const float* q,
+                                float* p, float* r, hipStream_t s);
+hipError_t launch_cg_beta_ref(float* scal, hipStream_t s);
+hipError_t launch_cg_dir_ref(const StepParams& P, const float* scal, const float* r, float* d,
+                             hipStream_t s);
+hipError_t launch_cg_iter_count_ref(const float* scal, uint32_t* count, hipStream_t s);
+
+// Mixed-precision solve state (device), f64 scalars.
+struct CgState {
+    double rs_old, rs_init, dq, rr;
+    double true_res_sq;
+    uint32_t iterations;
+    uint32_t active;
+    uint32_t error;          // nonzero: a grid barrier timed out
+    uint32_t pad;
+};
+struct CgWork {
+    // grid-barrier + reduction scratch
+    uint32_t* bar_count;     // [1]
+    uint32_t* bar_gen;       // [1]
+    uint32_t* abort_flag;    // [1]
+    double* partials;        // [max_blocks]
+    double* result;          // [4]
+    uint32_t* tile_flags;    // [C/64 + 1]
+    uint32_t* tile_scan;     // [C/64 + 1]
+    uint32_t* tiles;         // [C/64] active tile list
+    uint32_t* n_tiles;       // [1]
+    CgState* state;          // [1]
+};
+hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uint8_t* ct,
+                             const uint32_t* occ, float* div, float* b, float* p, float* r, float* d,
+                             const CgWork& w, uint32_t* scan_ws, hipStream_t s);
+hipError_t launch_cg_persistent(const StepParams& P, const uint8_t* ct, const float* b, float* p,
+                                float* r, float* d0, float* d1, float* q, const CgWork& w,
+                                int grid_blocks, hipStream_t s);
+int cg_persistent_max_blocks();
+
+// ---- k_g2p.hip ----
+hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16,
+                      const float* const vel[3], const float* const saved[3], hipStream_t s);
+hipError_t launch_impulse(const StepParams& P, const ParticleSoA& live, int bf16, float dx, float dy,
+                          float dz, hipStream_t s);
+
+}  // namespace fs

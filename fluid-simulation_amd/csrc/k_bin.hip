@@ -1,0 +1,363 @@
+// k_bin.hip — particle binning: clears, cell keys + occupancy histogram,
+// exclusive scans and the stable counting-sort permutation.
+//
+// Reference: sh/mark.wgsl:27-45 (histogram), sh/sort_scan_block|spine|add.wgsl
+// (exclusive scan of the histogram = bin starts), sh/sort_scatter.wgsl:35-54
+// (permutation).  The reference's permutation claims slots with atomicAdd and
+// so orders particles inside a bin arbitrarily; this file produces the stable
+// order of the host mirror src/lib.rs:1188-1211 with an LSD radix sort on the
+// cell key (each 8-bit digit pass is stable), so the permutation is bit-exact
+// and deterministic.
+#include "fs_kernels.h"
+
+namespace fs {
+
+namespace {
+constexpr int kBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;     // 2048 elements per scan block
+constexpr int kRadixItems = 16;                    // keys per lane
+constexpr int kRadixWaves = kBlock / 64;
+constexpr int kRadixTile = kBlock * kRadixItems;   // 4096 keys per radix block
+constexpr int kRadixBins = 256;
+
+inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+}  // namespace
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, float v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// Fill that runs only when *guard != 0 (used for the P2G fallback path).
+__global__ void k_fill_u32_guarded(uint32_t* __restrict__ p, uint32_t v, size_t n, const uint32_t* guard) {
+    if (*guard == 0u) return;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u32_guarded(uint32_t* p, uint32_t value, size_t count, const uint32_t* guard,
+                                   hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned g = grid_for(count, kBlock);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_fill_u32_guarded, dim3(g), dim3(kBlock), 0, s, p, value, count, guard);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned g = grid_for(count, kBlock);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, float value, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned g = grid_for(count, kBlock);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_fill_f32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+// sh/mark.wgsl:27-45: one atomicAdd per particle.
+__global__ void k_mark(StepParams P, const float* __restrict__ px, const float* __restrict__ py,
+                       const float* __restrict__ pz, uint32_t* __restrict__ occ) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.N) return;
+    atomicAdd(&occ[cell_key(P, px[p], py[p], pz[p])], 1u);
+}
+
+hipError_t launch_mark(const StepParams& P, const float* const pos[3], uint32_t* occ, hipStream_t s) {
+    if (P.N == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mark, dim3(grid_for(P.N, 64)), dim3(64), 0, s, P, pos[0], pos[1], pos[2], occ);
+    return hipGetLastError();
+}
+
+// Keys + histogram.  Lanes holding equal consecutive keys (the common case once
+// particles are sorted) are merged into one atomic per run; the counts are
+// integers, so the result equals one atomic per particle exactly.
+__global__ void k_key_hist(StepParams P, const float* __restrict__ px, const float* __restrict__ py,
+                           const float* __restrict__ pz, uint32_t* __restrict__ keys,
+                           uint32_t* __restrict__ occ, uint32_t* __restrict__ misfit) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < P.N;
+    uint32_t key = 0xffffffffu;
+    bool mis = false;
+    if (valid) {
+        const float x = px[p], y = py[p], z = pz[p];
+        key = cell_key(P, x, y, z);
+        keys[p] = key;
+        // The P2G cell (sh/scatter.wgsl:94: multiply by inv_h, no clamp) must be
+        // the sort cell for the brick gather to see every tap.
+        const uint32_t ki = key % P.nx, kj = (key / P.nx) % P.ny, kk = key / (P.nx * P.ny);
+        mis = floorf((x - P.origin[0]) * P.inv_h) != (float)ki || floorf((y - P.origin[1]) * P.inv_h) != (float)kj ||
+              floorf((z - P.origin[2]) * P.inv_h) != (float)kk;
+    }
+    const uint64_t mm = __ballot(mis);
+    if (lane_id() == 0 && mm) atomicAdd(misfit, (uint32_t)__popcll(mm));
+    const uint32_t lane = lane_id();
+    uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t act = __ballot(valid);
+    if (head) {
+        uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)(64 - __builtin_clzll(act));
+        atomicAdd(&occ[key], end - lane);
+    }
+}
+
+hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys,
+                           uint32_t* occ, uint32_t* misfit, hipStream_t s) {
+    if (P.N == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_key_hist, dim3(grid_for(P.N, kBlock)), dim3(kBlock), 0, s, P, pos[0], pos[1],
+                       pos[2], keys, occ, misfit);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (u32, wrapping adds): tiles of 2048, tile sums scanned
+// recursively, then added back.  Integer prefix sums have one answer, so the
+// result equals the three WGSL scan passes exactly.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_wave[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) {
+        uint32_t x = s_wave[i];
+        if (i < w) base += x;
+        tot += x;
+    }
+    total = tot;
+    return base + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ tile_sums, size_t len) {
+    __shared__ uint32_t s_data[kScanTile];
+    __shared__ uint32_t s_wave[kBlock / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    for (int i = threadIdx.x; i < kScanTile; i += kBlock) {
+        size_t g = base + i;
+        s_data[i] = g < len ? in[g] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        loc[i] = sum;
+        sum += s_data[threadIdx.x * kScanItems + i];
+    }
+    uint32_t total;
+    uint32_t pre = block_excl_scan(sum, s_wave, total);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) s_data[threadIdx.x * kScanItems + i] = pre + loc[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < kScanTile; i += kBlock) {
+        size_t g = base + i;
+        if (g < len) out[g] = s_data[i];
+    }
+    if (tile_sums && threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_off, size_t len) {
+    size_t g = (size_t)blockIdx.x * kScanTile + threadIdx.x;
+    const uint32_t add = tile_off[blockIdx.x];
+    for (int i = 0; i < kScanItems; ++i, g += kBlock)
+        if (g < len) out[g] += add;
+}
+
+size_t scan_workspace(size_t len) {
+    size_t total = 0;
+    while (len > (size_t)kScanTile) {
+        len = (len + kScanTile - 1) / kScanTile;
+        total += 2 * len;
+    }
+    return total + 16;
+}
+
+hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
+                                 hipStream_t s) {
+    if (len == 0) return hipSuccess;
+    size_t tiles = (len + kScanTile - 1) / kScanTile;
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, in, out, (uint32_t*)nullptr, len);
+        return hipGetLastError();
+    }
+    uint32_t* sums = ws;
+    uint32_t* sums_scanned = ws + tiles;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, out, sums, len);
+    hipError_t e = launch_exclusive_scan(sums, sums_scanned, tiles, ws + 2 * tiles, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kBlock), 0, s, out, sums_scanned, len);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort over 8-bit digits.
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                        uint32_t shift, uint32_t ntiles,
+                                                        uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[kRadixBins];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kRadixTile;
+#pragma unroll 4
+    for (int i = 0; i < kRadixItems; ++i) {
+        size_t idx = base + threadIdx.x + (size_t)i * kBlock;
+        if (idx < n) atomicAdd(&s_h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// Each wave ranks 1024 consecutive keys in 16 rounds of 64; equal digits inside
+// a round are found with eight ballots.  Order of items = (wave, round, lane)
+// = input order, so equal digits keep their input order.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    const uint32_t* __restrict__ offsets, uint32_t n, uint32_t shift, uint32_t ntiles,
+    int write_keys) {
+    __shared__ uint32_t s_cnt[kRadixWaves][kRadixBins];
+    for (int i = threadIdx.x; i < kRadixWaves * kRadixBins; i += kBlock) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    const size_t base = (size_t)blockIdx.x * kRadixTile + (size_t)w * (kRadixTile / kRadixWaves);
+    uint32_t kk[kRadixItems], vv[kRadixItems], rk[kRadixItems];
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        size_t idx = base + (size_t)r * 64 + lane;
+        bool valid = idx < n;
+        uint32_t key = valid ? keys_in[idx] : 0u;
+        uint32_t val = valid ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
+        uint32_t digit = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (digit >> b) & 1u;
+            uint64_t m = __ballot(valid && bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        uint32_t old = s_cnt[w][digit];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) s_cnt[w][digit] = old + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        kk[r] = key;
+        vv[r] = val;
+        rk[r] = valid ? old + rank : 0xffffffffu;
+    }
+    __syncthreads();
+    {
+        // digit t: exclusive offsets over waves + global base of (digit, tile).
+        const uint32_t t = threadIdx.x;
+        uint32_t run = offsets[(size_t)t * ntiles + blockIdx.x];
+        for (int ww = 0; ww < kRadixWaves; ++ww) {
+            uint32_t c = s_cnt[ww][t];
+            s_cnt[ww][t] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        if (rk[r] == 0xffffffffu) continue;
+        uint32_t pos = s_cnt[w][(kk[r] >> shift) & 255u] + rk[r];
+        if (write_keys) keys_out[pos] = kk[r];
+        vals_out[pos] = vv[r];
+    }
+}
+
+size_t radix_workspace(size_t n) {
+    size_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    size_t table = ntiles * kRadixBins;
+    return 2 * table + scan_workspace(table) + 16;
+}
+
+hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t key_limit,
+                                     uint32_t* keys_tmp0, uint32_t* keys_tmp1, uint32_t* vals_tmp,
+                                     uint32_t* perm, uint32_t* ws, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t bits = 1;
+    while (bits < 32 && (1ull << bits) < (uint64_t)key_limit) ++bits;
+    const uint32_t passes = (bits + 7) / 8;
+    const uint32_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    const size_t table = (size_t)ntiles * kRadixBins;
+    uint32_t* hist = ws;
+    uint32_t* offs = ws + table;
+    uint32_t* scan_ws = ws + 2 * table;
+    // Value buffers: the last pass must land in `perm`.
+    const uint32_t* kin = keys;
+    const uint32_t* vin = nullptr;
+    uint32_t* kbuf[2] = {keys_tmp0, keys_tmp1};
+    uint32_t* vbuf[2];
+    if (passes % 2 == 1) { vbuf[0] = perm; vbuf[1] = vals_tmp; }
+    else { vbuf[0] = vals_tmp; vbuf[1] = perm; }
+    for (uint32_t ps = 0; ps < passes; ++ps) {
+        const uint32_t shift = 8 * ps;
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+        hipError_t e = launch_exclusive_scan(hist, offs, table, scan_ws, s);
+        if (e != hipSuccess) return e;
+        uint32_t* kout = kbuf[ps & 1];
+        uint32_t* vout = vbuf[ps & 1];
+        const int write_keys = ps + 1 < passes;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+                           n, shift, ntiles, write_keys);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        kin = kout;
+        vin = vout;
+    }
+    return hipSuccess;
+}
+
+template <int BF16>
+__global__ void k_permute(uint32_t n, const uint32_t* __restrict__ perm, ParticleSoA src, ParticleSoA dst) {
+    uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const uint32_t sidx = perm[d];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dst.pos[a][d] = src.pos[a][sidx];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (BF16) ((uint16_t*)dst.vel[a])[d] = ((const uint16_t*)src.vel[a])[sidx];
+        else ((float*)dst.vel[a])[d] = ((const float*)src.vel[a])[sidx];
+    }
+}
+
+hipError_t launch_permute(uint32_t n, const uint32_t* perm, const ParticleSoA& src, const ParticleSoA& dst,
+                          int bf16, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (bf16) hipLaunchKernelGGL(k_permute<1>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, perm, src, dst);
+    else hipLaunchKernelGGL(k_permute<0>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, perm, src, dst);
+    return hipGetLastError();
+}
+
+}  // namespace fs

@@ -1,0 +1,332 @@
+"""Python front end of libfluidstep.so (include/fluidstep.h) via ctypes.
+
+`FluidStep` mirrors fluid-lab's `GpuFluid` (app/crates/fluid-lab/src/gpu/
+fluid.rs:100-1716): construct from parameters and particles, `step(n)`
+(GpuContext::step, src/gpu/mod.rs:950-989), `reset()`, live `set_param(id, v)`
+with the registry's clamp/apply semantics (src/settings/mod.rs:1733-1821),
+`set_gravity`, `apply_impulse`, and read-outs of the step's buffers.
+
+There is no CPU fallback: loading fails loudly if the library is missing, and
+creating a tank fails with FluidStepError when no HIP device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfluidstep.so")
+
+FS_OK = 0
+ENGINE_REFERENCE, ENGINE_FUSED = 0, 1
+PRECISION_F32, PRECISION_MIXED = 0, 1
+VELOCITY_F32, VELOCITY_BF16 = 0, 1
+FACE_VELOCITY, FACE_SAVED = 0, 1
+CELL_PRESSURE, CELL_DIVERGENCE = 0, 1
+CELL_TYPE, CELL_OCCUPANCY, CELL_OFFSET = 0, 1, 2
+STATUS_NAMES = {0: "applied", 1: "stored", 2: "unknown_id", 3: "non_finite_rejected"}
+APPLY_NAMES = {0: "live", 1: "reset", 2: "reload", -1: None}
+
+EXPORTED = (
+    "fs_registry_create", "fs_registry_destroy", "fs_registry_set", "fs_registry_get",
+    "fs_registry_count", "fs_registry_id", "fs_registry_to_params", "fs_params_default",
+    "fs_create", "fs_destroy", "fs_reset", "fs_load_state", "fs_step", "fs_synchronize",
+    "fs_set_timing", "fs_set_param", "fs_set_gravity", "fs_apply_impulse",
+    "fs_read_particles", "fs_read_faces", "fs_read_cells_f32", "fs_read_cells_u32",
+    "fs_read_permutation", "fs_get_stats", "fs_get_params", "fs_particle_count",
+    "fs_buffer_bytes", "fs_last_error", "fs_version",
+)
+
+
+class FluidStepError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fluidstep error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("h", C.c_float), ("origin", C.c_float * 3), ("dt", C.c_float),
+        ("gravity", C.c_float * 3), ("rho", C.c_float),
+        ("flip_blend", C.c_float), ("wall_friction", C.c_float), ("rest_density", C.c_float),
+        ("volume_stiffness", C.c_float), ("drift_clamp", C.c_float),
+        ("liquid_threshold", C.c_uint32), ("surface_dilation", C.c_uint32), ("cfl", C.c_float),
+        ("pressure_iterations", C.c_uint32), ("pressure_warm_start", C.c_uint32),
+        ("pressure_residual_tolerance", C.c_float),
+        ("particle_sort", C.c_uint32), ("particle_sort_period", C.c_uint32),
+        ("engine", C.c_uint32), ("precision", C.c_uint32), ("velocity_storage", C.c_uint32),
+        ("scene_density", C.c_float), ("reserved", C.c_uint32 * 4),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class Mutation(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("accepted", C.c_int32), ("clamped", C.c_int32),
+        ("apply_class", C.c_int32), ("applied_live", C.c_int32), ("needs_reset", C.c_int32),
+        ("requested_value", C.c_double), ("stored_value", C.c_double),
+    ]
+
+    def as_dict(self):
+        stored = None if math.isnan(self.stored_value) else self.stored_value
+        return {
+            "ok": bool(self.accepted), "status": STATUS_NAMES[self.status],
+            "stored_value": stored, "clamped": bool(self.clamped),
+            "apply": APPLY_NAMES[self.apply_class], "applied_live": bool(self.applied_live),
+            "needs_reset": bool(self.needs_reset),
+        }
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("liquid_cells", C.c_uint32), ("cg_iterations", C.c_uint32),
+        ("sorted", C.c_uint32), ("p2g_gather", C.c_uint32),
+        ("rs_initial", C.c_double), ("rs_final", C.c_double), ("true_residual_sq", C.c_double),
+        ("ms_total", C.c_double), ("ms_prep", C.c_double), ("ms_pressure", C.c_double),
+        ("ms_finish", C.c_double), ("substeps", C.c_uint64),
+        ("ms_cg", C.c_double), ("cg_active_cells", C.c_uint32), ("misfits", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libfluidstep.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python fluid-simulation_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.fs_registry_create.restype = vp
+        L.fs_registry_destroy.argtypes = [vp]
+        L.fs_registry_set.argtypes = [vp, C.c_char_p, C.c_double, C.POINTER(Mutation)]
+        L.fs_registry_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double)]
+        L.fs_registry_count.argtypes = [vp]
+        L.fs_registry_id.argtypes = [vp, C.c_int]
+        L.fs_registry_id.restype = C.c_char_p
+        L.fs_registry_to_params.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_float, C.POINTER(Params)]
+        L.fs_params_default.argtypes = [C.POINTER(Params)]
+        L.fs_create.argtypes = [C.POINTER(Params), vp, vp, C.c_uint32, C.POINTER(vp)]
+        L.fs_destroy.argtypes = [vp]
+        L.fs_reset.argtypes = [vp]
+        L.fs_load_state.argtypes = [vp, vp, vp, vp]
+        L.fs_step.argtypes = [vp, C.c_uint32]
+        L.fs_synchronize.argtypes = [vp]
+        L.fs_set_timing.argtypes = [vp, C.c_int]
+        L.fs_set_param.argtypes = [vp, C.c_char_p, C.c_double, C.POINTER(Mutation)]
+        L.fs_set_gravity.argtypes = [vp, C.c_float, C.c_float, C.c_float]
+        L.fs_apply_impulse.argtypes = [vp, C.c_float, C.c_float, C.c_float]
+        L.fs_read_particles.argtypes = [vp, vp, vp]
+        L.fs_read_faces.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.fs_read_cells_f32.argtypes = [vp, C.c_int, vp]
+        L.fs_read_cells_u32.argtypes = [vp, C.c_int, vp]
+        L.fs_read_permutation.argtypes = [vp, vp]
+        L.fs_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.fs_get_params.argtypes = [vp, C.POINTER(Params)]
+        L.fs_particle_count.argtypes = [vp]
+        L.fs_particle_count.restype = C.c_uint32
+        L.fs_buffer_bytes.argtypes = [vp]
+        L.fs_buffer_bytes.restype = C.c_uint64
+        L.fs_last_error.restype = C.c_char_p
+        L.fs_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != FS_OK:
+        raise FluidStepError(rc, lib().fs_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def default_params() -> Params:
+    p = Params()
+    _check(lib().fs_params_default(C.byref(p)))
+    return p
+
+
+def make_params(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), **kw) -> Params:
+    """Parameters with the reference registry defaults overridden by kw."""
+    p = default_params()
+    p.nx, p.ny, p.nz, p.h = nx, ny, nz, h
+    for a in range(3):
+        p.origin[a] = origin[a]
+    for k, v in kw.items():
+        if k == "gravity":
+            for a in range(3):
+                p.gravity[a] = v[a]
+        else:
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+    return p
+
+
+class Registry:
+    """The settings registry (host only)."""
+
+    def __init__(self):
+        self._h = lib().fs_registry_create()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fs_registry_destroy(self._h)
+            self._h = None
+
+    def set(self, id_: str, value: float) -> dict:
+        m = Mutation()
+        _check(lib().fs_registry_set(self._h, id_.encode(), float(value), C.byref(m)))
+        return m.as_dict()
+
+    def get(self, id_: str) -> float:
+        v = C.c_double()
+        _check(lib().fs_registry_get(self._h, id_.encode(), C.byref(v)))
+        return v.value
+
+    def ids(self):
+        n = lib().fs_registry_count(self._h)
+        return [lib().fs_registry_id(self._h, i).decode() for i in range(n)]
+
+    def to_params(self, h, origin, scene_density) -> Params:
+        p = Params()
+        o = (C.c_float * 3)(*origin)
+        _check(lib().fs_registry_to_params(self._h, h, o, scene_density, C.byref(p)))
+        return p
+
+
+class FluidStep:
+    """One tank on the GPU (the GpuFluid drop-in)."""
+
+    def __init__(self, params: Params, pos, vel=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        h = C.c_void_p()
+        _check(lib().fs_create(C.byref(params), _ptr(pos), _ptr(vel), C.c_uint32(len(pos)), C.byref(h)))
+        self._h = h
+        self.params = params
+        self.n = len(pos)
+        self.dims = (params.nx, params.ny, params.nz)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fs_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # lifecycle
+    def step(self, substeps: int = 1):
+        _check(lib().fs_step(self._h, C.c_uint32(substeps)))
+
+    def synchronize(self):
+        _check(lib().fs_synchronize(self._h))
+
+    def reset(self):
+        _check(lib().fs_reset(self._h))
+
+    def load_state(self, pos, vel=None, pressure=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        pr = None if pressure is None else np.ascontiguousarray(pressure, dtype=np.float32)
+        _check(lib().fs_load_state(self._h, _ptr(pos), _ptr(vel), _ptr(pr)))
+
+    def set_timing(self, on: bool):
+        _check(lib().fs_set_timing(self._h, int(on)))
+
+    # setters
+    def set_param(self, id_: str, value: float) -> dict:
+        m = Mutation()
+        _check(lib().fs_set_param(self._h, id_.encode(), float(value), C.byref(m)))
+        return m.as_dict()
+
+    def set_gravity(self, gx, gy, gz):
+        _check(lib().fs_set_gravity(self._h, gx, gy, gz))
+
+    def apply_impulse(self, dx, dy, dz):
+        _check(lib().fs_apply_impulse(self._h, dx, dy, dz))
+
+    # read-outs
+    def face_count(self, a):
+        d = list(self.dims)
+        d[a] += 1
+        return d[0] * d[1] * d[2]
+
+    @property
+    def cell_count(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def particles(self):
+        pos = np.zeros((self.n, 3), np.float32)
+        vel = np.zeros((self.n, 3), np.float32)
+        _check(lib().fs_read_particles(self._h, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def faces(self, which=FACE_VELOCITY):
+        out = []
+        for a in range(3):
+            f = np.zeros(self.face_count(a), np.float32)
+            _check(lib().fs_read_faces(self._h, a, which, _ptr(f)))
+            out.append(f)
+        return out
+
+    def pressure(self):
+        p = np.zeros(self.cell_count, np.float32)
+        _check(lib().fs_read_cells_f32(self._h, CELL_PRESSURE, _ptr(p)))
+        return p
+
+    def divergence(self):
+        p = np.zeros(self.cell_count, np.float32)
+        _check(lib().fs_read_cells_f32(self._h, CELL_DIVERGENCE, _ptr(p)))
+        return p
+
+    def cells_u32(self, which):
+        p = np.zeros(self.cell_count, np.uint32)
+        _check(lib().fs_read_cells_u32(self._h, which, _ptr(p)))
+        return p
+
+    def cell_type(self):
+        return self.cells_u32(CELL_TYPE)
+
+    def occupancy(self):
+        return self.cells_u32(CELL_OCCUPANCY)
+
+    def cell_offset(self):
+        return self.cells_u32(CELL_OFFSET)
+
+    def permutation(self):
+        p = np.zeros(self.n, np.uint32)
+        _check(lib().fs_read_permutation(self._h, _ptr(p)))
+        return p
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().fs_get_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def buffer_bytes(self) -> int:
+        return int(lib().fs_buffer_bytes(self._h))

@@ -1,0 +1,188 @@
+/*
+ * fluidstep.h — C-ABI of the MI355X FLIP/PIC simulation step.
+ *
+ * The drop-in boundary is fluid-lab's `GpuFluid` (app/crates/fluid-lab/src/
+ * gpu/fluid.rs:100-1716) together with the live-setting path of its registry
+ * (src/settings/mod.rs:1733-1821) and `GpuContext::step`
+ * (src/gpu/mod.rs:950-1033).  Each entry point below names the reference
+ * interface it replaces.  Conventions: plain pointers and sizes; `int` status
+ * codes (FS_OK == 0, negative on error, message in fs_last_error()); no
+ * exceptions or panics cross the ABI; the caller owns every host array; a
+ * handle owns its device buffers and HIP stream and is not thread-safe;
+ * read-outs reflect the state after the last completed fs_step/fs_reset.
+ * The product library needs a gfx950 GPU: without one, fs_create fails with
+ * FS_ERR_DEVICE (there is no CPU fallback).
+ */
+#ifndef FLUIDSTEP_H
+#define FLUIDSTEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define FS_API __declspec(dllexport)
+#else
+#define FS_API __attribute__((visibility("default")))
+#endif
+
+/* ---- status codes ---- */
+#define FS_OK 0
+#define FS_ERR_INVALID_ARGUMENT (-1)
+#define FS_ERR_UNSUPPORTED (-2)
+#define FS_ERR_DEVICE (-3)
+#define FS_ERR_OUT_OF_MEMORY (-4)
+#define FS_ERR_NOT_FINITE (-5)
+#define FS_ERR_SOLVER (-6)
+
+/* ---- step configuration ---- */
+#define FS_ENGINE_REFERENCE 0 /* one kernel per WGSL pass, host-driven CG loop */
+#define FS_ENGINE_FUSED 1     /* fused passes, sorted-segment P2G, persistent CG */
+#define FS_PRECISION_F32 0    /* f32 everywhere, the WGSL f32 reduction trees */
+#define FS_PRECISION_MIXED 1  /* f32 vectors, f64 dots / scalars / true residual */
+#define FS_VELOCITY_F32 0
+#define FS_VELOCITY_BF16 1    /* particle velocity stored as bf16, math in f32 */
+
+/* Creation parameters: the reference Params uniform (src/gpu/fluid.rs:25-39,
+ * filled at :289-318) with every field explicit. */
+typedef struct fs_params {
+    uint32_t nx, ny, nz;          /* grid.res_x/y/z */
+    float h;                      /* uniform cell size (sim::H in the reference) */
+    float origin[3];              /* world position of cell (0,0,0)'s low corner */
+    float dt;                     /* physics.fixed_dt */
+    float gravity[3];             /* grav (set_gravity_vec) */
+    float rho;                    /* 1000 in the reference */
+    float flip_blend;             /* physics.flip_blend */
+    float wall_friction;          /* physics.wall_friction */
+    float rest_density;           /* effective rest packing (particles per cell) */
+    float volume_stiffness;       /* physics.volume_stiffness */
+    float drift_clamp;            /* physics.drift_clamp */
+    uint32_t liquid_threshold;    /* classify.liquid_threshold */
+    uint32_t surface_dilation;    /* effective classify.surface_dilation */
+    float cfl;                    /* physics.cfl */
+    uint32_t pressure_iterations; /* solver.pressure_iterations (CG cap) */
+    uint32_t pressure_warm_start; /* solver.pressure_warm_start */
+    float pressure_residual_tolerance; /* solver.pressure_residual_tolerance */
+    uint32_t particle_sort;       /* dev.particle_sort */
+    uint32_t particle_sort_period;/* dev.particle_sort_period */
+    uint32_t engine;              /* FS_ENGINE_* */
+    uint32_t precision;           /* FS_PRECISION_* */
+    uint32_t velocity_storage;    /* FS_VELOCITY_* */
+    float scene_density;          /* particles per seeded cell, for Auto rest/dilation */
+    uint32_t reserved[4];
+} fs_params;
+
+/* Result of a setting mutation (SettingMutationResult, src/settings/mod.rs:43-127). */
+#define FS_STATUS_APPLIED 0
+#define FS_STATUS_STORED 1
+#define FS_STATUS_UNKNOWN_ID 2
+#define FS_STATUS_NON_FINITE_REJECTED 3
+#define FS_APPLY_LIVE 0
+#define FS_APPLY_RESET 1
+#define FS_APPLY_RELOAD 2
+typedef struct fs_mutation {
+    int32_t status;        /* FS_STATUS_* */
+    int32_t accepted;      /* status is neither unknown_id nor non_finite_rejected */
+    int32_t clamped;
+    int32_t apply_class;   /* FS_APPLY_*, -1 when unknown */
+    int32_t applied_live;  /* the step's parameters changed now */
+    int32_t needs_reset;
+    double requested_value;
+    double stored_value;   /* NaN when nothing was stored */
+} fs_mutation;
+
+/* Per-step diagnostics (stats buffer, cg_scalars, timing sections). */
+typedef struct fs_stats {
+    uint32_t liquid_cells;     /* stats[0], sh/classify.wgsl:55-57 */
+    uint32_t cg_iterations;    /* iterations run while the solve was active */
+    uint32_t sorted;           /* the last substep re-sorted the particles */
+    uint32_t p2g_gather;       /* the last substep used the sorted-segment gather */
+    double rs_initial;         /* cg_scalars[4] */
+    double rs_final;           /* cg_scalars[0] after the loop */
+    double true_residual_sq;   /* |b - A p|^2 in f64 (mixed precision), else NaN */
+    double ms_total;           /* device time of the last fs_step call (timing on) */
+    double ms_prep;            /* clear, mark, sort, classify, P2G epilogue */
+    double ms_pressure;        /* divergence + CG */
+    double ms_finish;          /* gradient + enforce + G2P */
+    uint64_t substeps;         /* substeps run since create/reset */
+    double ms_cg;              /* the CG iterations alone (fused: the persistent kernel) */
+    uint32_t cg_active_cells;  /* cells in 64-cell tiles holding liquid (fused engine) */
+    uint32_t misfits;          /* particles whose P2G cell differs from their sort cell */
+} fs_stats;
+
+typedef struct fs_tank fs_tank;
+typedef struct fs_registry fs_registry;
+
+/* ---- settings registry (host only; no GPU needed) ---- */
+/* Registry::default (src/settings/mod.rs:404) restricted to the step's rows. */
+FS_API fs_registry* fs_registry_create(void);
+FS_API void fs_registry_destroy(fs_registry* r);
+/* Registry::set_value_f64_result (src/settings/mod.rs:1733-1821). */
+FS_API int fs_registry_set(fs_registry* r, const char* id, double value, fs_mutation* out);
+FS_API int fs_registry_get(const fs_registry* r, const char* id, double* value);
+FS_API int fs_registry_count(const fs_registry* r);
+FS_API const char* fs_registry_id(const fs_registry* r, int index);
+/* GpuFluid::new's Params fill (src/gpu/fluid.rs:258-318) from registry values:
+ * h, origin and scene_density come from the caller. */
+FS_API int fs_registry_to_params(const fs_registry* r, float h, const float origin[3], float scene_density,
+                                 fs_params* out);
+
+/* ---- step lifecycle ---- */
+/* Defaults of the reference registry (80x40x80, H = 2/64, centred origin). */
+FS_API int fs_params_default(fs_params* p);
+/* GpuFluid::new (src/gpu/fluid.rs:247-1197): allocate every buffer and upload
+ * the initial particles.  pos_xyz / vel_xyz hold n xyz triplets (vel may be
+ * NULL = zero).  Non-finite inputs are rejected. */
+FS_API int fs_create(const fs_params* p, const float* pos_xyz, const float* vel_xyz, uint32_t n, fs_tank** out);
+FS_API int fs_destroy(fs_tank* t);
+/* GpuFluid::reset (src/gpu/fluid.rs:1408-1418): initial particles back,
+ * pressure zeroed, sort tick 0. */
+FS_API int fs_reset(fs_tank* t);
+/* Overwrite the live state (particles, and the warm-start pressure when not
+ * NULL); used to start a step from a given state.  No reference counterpart. */
+FS_API int fs_load_state(fs_tank* t, const float* pos_xyz, const float* vel_xyz, const float* pressure);
+/* GpuContext::step(n) (src/gpu/mod.rs:950-989): n substeps in the order of
+ * record_substep_coarse (src/gpu/mod.rs:993-1033).  Returns after enqueueing;
+ * read-outs and fs_synchronize wait. */
+FS_API int fs_step(fs_tank* t, uint32_t substeps);
+FS_API int fs_synchronize(fs_tank* t);
+/* Record per-section device times into fs_stats (GpuTimers, src/gpu/timing.rs). */
+FS_API int fs_set_timing(fs_tank* t, int enabled);
+
+/* ---- live setters ---- */
+/* FluidApp::mutate_setting + apply_live_setting (src/lib.rs:680-867) ->
+ * GpuFluid::set_* (src/gpu/fluid.rs:1295-1371). */
+FS_API int fs_set_param(fs_tank* t, const char* id, double value, fs_mutation* out);
+/* GpuFluid::set_gravity_vec (src/gpu/fluid.rs:1347-1350). */
+FS_API int fs_set_gravity(fs_tank* t, float gx, float gy, float gz);
+/* GpuFluid::apply_impulse (src/gpu/fluid.rs:1374-1395). */
+FS_API int fs_apply_impulse(fs_tank* t, float dx, float dy, float dz);
+
+/* ---- read-outs (GpuFluid::*_buffer accessors, src/gpu/fluid.rs:1199-1292) ---- */
+#define FS_FACE_VELOCITY 0 /* u/v/w_vel after the step */
+#define FS_FACE_SAVED 1    /* u/v/w_saved: post-P2G, pre-force */
+#define FS_CELL_PRESSURE 0
+#define FS_CELL_DIVERGENCE 1
+#define FS_CELL_TYPE 0      /* 0 Solid, 1 Liquid, 2 Air */
+#define FS_CELL_OCCUPANCY 1 /* particles per cell (mark) */
+#define FS_CELL_OFFSET 2    /* bin starts of the last sort */
+FS_API int fs_read_particles(fs_tank* t, float* pos_xyz, float* vel_xyz);
+FS_API int fs_read_faces(fs_tank* t, int axis, int which, float* out);
+FS_API int fs_read_cells_f32(fs_tank* t, int which, float* out);
+FS_API int fs_read_cells_u32(fs_tank* t, int which, uint32_t* out);
+/* perm[d] = the source index of sorted slot d in the last sort. */
+FS_API int fs_read_permutation(fs_tank* t, uint32_t* perm);
+FS_API int fs_get_stats(fs_tank* t, fs_stats* out);
+FS_API int fs_get_params(const fs_tank* t, fs_params* out);
+FS_API uint32_t fs_particle_count(const fs_tank* t);
+FS_API uint64_t fs_buffer_bytes(const fs_tank* t);
+/* Thread-local message of the last failing call on this thread. */
+FS_API const char* fs_last_error(void);
+FS_API const char* fs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

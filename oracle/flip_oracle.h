@@ -1,0 +1,143 @@
+/*
+ * flip_oracle.h — CPU restatement of fluid-lab's FLIP/PIC substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity checker for the HIP step in
+ * fluid-simulation_amd/ and the timed CPU baseline in bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product library never links or calls it.
+ *
+ * The same source is compiled twice: with ORC_REAL=float (suffix _f32, the
+ * plain-FP32 restatement that mirrors the WGSL passes operation by operation)
+ * and with ORC_REAL=double (suffix _f64, the FP64 oracle used for the precision
+ * reports).  Every function cites the reference file:line it restates; the
+ * citations use the shorthand
+ *     sh/   = app/crates/fluid-lab/src/gpu/shaders/
+ *     src/  = app/crates/fluid-lab/src/
+ *
+ * Parity pins (tests/test_oracle_*.py): the reference's own known-answer tests
+ * (src/sim/mod.rs:187-602, src/sim/pressure.rs:513-783, src/lib.rs:1232-1300)
+ * plus closed forms (partition of unity, uniform-field reproduction, dense
+ * solve of the Poisson operator, SPD checks, CFL/wall clamps).  Functions that
+ * none of these pins cover are marked "parity unpinned" below.
+ */
+#ifndef FLIP_ORACLE_H
+#define FLIP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Step constants: the reference's 128-byte Params uniform
+ * (src/gpu/fluid.rs:25-39) unpacked into named fields.  All float fields are
+ * f32, exactly as the uniform holds them; the f64 oracle promotes them. */
+typedef struct orc_params {
+    uint32_t nx, ny, nz;
+    uint32_t pressure_iters;      /* dims.z: fixed CG launch count (>= 1)            */
+    uint32_t warm_start;          /* dims.w                                         */
+    uint32_t reduce_mode;         /* 0 = WGSL f32 256-tree, 1 = f64 dots (mixed)    */
+    float h, inv_h;               /* geom.x, geom.y (inv_h = f32(1/h))              */
+    float dt, fixed_scale;        /* geom.z, geom.w (2^16)                          */
+    float rho, flip_blend, wall_friction;   /* phys.y, phys.z, phys.w               */
+    float origin[3];
+    float grav[3];
+    float rest, stiff, drift_clamp;         /* spc.xyz                              */
+    float liquid_threshold, surface_dilation, cfl;  /* cls.xyz                      */
+    float tol_sq;                 /* cg_scalars[6] = clamp(tol,0,0.1)^2 (f32)       */
+    float tol;                    /* the unsquared clamped tolerance (mixed mode)   */
+} orc_params;
+
+/* Per-step diagnostics (stats buffer + cg_scalars of the reference). */
+typedef struct orc_stats {
+    uint32_t liquid_cells;        /* stats[0] (sh/classify.wgsl:55-57)              */
+    uint32_t cg_iterations;       /* iterations that ran while active               */
+    uint32_t misc;
+    uint32_t pad;
+    double rs_initial;            /* cg_scalars[4]                                  */
+    double rs_final;              /* cg_scalars[0] after the loop                   */
+    double true_residual_sq;      /* |b - A p|^2 in f64 after the solve             */
+} orc_stats;
+
+/* Optional outputs of one step; any pointer may be NULL.  Sizes:
+ * C = nx*ny*nz, Fu=(nx+1)ny nz, Fv=nx(ny+1)nz, Fw=nx ny(nz+1), N particles. */
+typedef struct orc_outputs_f32 {
+    uint32_t *occupancy, *cell_offset, *perm, *cell_type;   /* C, C, N, C */
+    int32_t *num[3], *den[3];                                /* per axis   */
+    float *vel[3], *saved[3];                                /* after P2G epilogue (pre-pressure) */
+    float *vel_final[3];                                     /* after gradient + enforce */
+    float *divergence;                                       /* C */
+    orc_stats *stats;
+} orc_outputs_f32;
+
+typedef struct orc_outputs_f64 {
+    uint32_t *occupancy, *cell_offset, *perm, *cell_type;
+    int64_t *num[3], *den[3];
+    double *vel[3], *saved[3];
+    double *vel_final[3];
+    double *divergence;
+    orc_stats *stats;
+} orc_outputs_f64;
+
+/* ---- binning (src/lib.rs:1161-1211, sh/mark.wgsl, sh/sort_*.wgsl) ---- */
+uint32_t orc_cell_key_f32(const orc_params *P, float x, float y, float z);
+void orc_histogram_f32(const orc_params *P, uint32_t n, const float *px, const float *py,
+                       const float *pz, uint32_t *occ);
+void orc_exclusive_scan_u32(uint32_t len, const uint32_t *in, uint32_t *out);
+/* Stable counting sort of keys[0..n): perm[d] = source index of sorted slot d. */
+void orc_stable_permutation(uint32_t n, uint32_t cells, const uint32_t *keys, uint32_t *perm);
+
+/* ---- cell typing (sh/classify.wgsl:25-61) ---- */
+uint32_t orc_classify_u32(const orc_params *P, const uint32_t *occ, uint32_t *cell_type);
+
+/* ---- P2G (sh/scatter.wgsl:44-118) ---- */
+void orc_p2g_f32(const orc_params *P, uint32_t n, const float *px, const float *py,
+                 const float *pz, const float *vx, const float *vy, const float *vz,
+                 int32_t *num[3], int32_t *den[3]);
+void orc_p2g_f64(const orc_params *P, uint32_t n, const double *px, const double *py,
+                 const double *pz, const double *vx, const double *vy, const double *vz,
+                 int64_t *num[3], int64_t *den[3]);
+
+/* ---- grid passes ---- */
+void orc_divergence_f32(const orc_params *P, float *const vel[3], const uint32_t *ct,
+                        const uint32_t *occ, float *div);
+void orc_apply_poisson_f32(const orc_params *P, const uint32_t *ct, const float *x, float *out);
+void orc_apply_poisson_f64(const orc_params *P, const uint32_t *ct, const double *x, double *out);
+/* CG pressure solve in the WGSL dispatch order (sh/cg_*.wgsl); p is in/out
+ * (warm-start state).  Returns iterations run while active. */
+uint32_t orc_pressure_solve_f32(const orc_params *P, const uint32_t *ct, const float *div,
+                                float *p, orc_stats *st);
+uint32_t orc_pressure_solve_f64(const orc_params *P, const uint32_t *ct, const double *div,
+                                double *p, orc_stats *st);
+void orc_gradient_enforce_f32(const orc_params *P, const float *p, const uint32_t *ct,
+                              float *vel[3]);
+/* Host-reference Jacobi (src/sim/pressure.rs:71-137), used by the KAT only. */
+void orc_jacobi_solve_f32(const orc_params *P, const uint32_t *ct, const float *div,
+                          uint32_t iters, float *p);
+
+/* ---- G2P (sh/g2p.wgsl:56-198) ---- */
+/* Wall-aware gather of (final, saved) for one axis at world point (x,y,z). */
+void orc_sample_axis_f32(const orc_params *P, int axis, float x, float y, float z,
+                         const float *vel, const float *saved, int wall_aware,
+                         float *out_fin, float *out_sav);
+void orc_g2p_f32(const orc_params *P, uint32_t n, float *px, float *py, float *pz,
+                 float *vx, float *vy, float *vz, float *const vel[3], float *const saved[3]);
+
+/* ---- the full substep (src/gpu/mod.rs:993-1033 order) ---- */
+/* Runs `steps` substeps in place.  do_sort: re-sort on every substep where
+ * (tick % sort_period == 0); tick0 is the sort tick of the first substep.
+ * Outputs describe the LAST substep. */
+int orc_run_f32(const orc_params *P, uint32_t n, float *px, float *py, float *pz,
+                float *vx, float *vy, float *vz, float *pressure, uint32_t steps,
+                uint32_t sort_period, uint32_t tick0, orc_outputs_f32 *out);
+int orc_run_f64(const orc_params *P, uint32_t n, double *px, double *py, double *pz,
+                double *vx, double *vy, double *vz, double *pressure, uint32_t steps,
+                uint32_t sort_period, uint32_t tick0, orc_outputs_f64 *out);
+
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

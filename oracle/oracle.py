@@ -1,0 +1,322 @@
+"""ctypes front end of the CPU restatement in oracle/ (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product library in
+fluid-simulation_amd/ never imports this module.
+
+Each wrapper names the C function it calls; the C sources cite the reference
+file:line they restate (app/crates/fluid-lab/src/...).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_u32p = C.POINTER(C.c_uint32)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("pressure_iters", C.c_uint32), ("warm_start", C.c_uint32), ("reduce_mode", C.c_uint32),
+        ("h", C.c_float), ("inv_h", C.c_float), ("dt", C.c_float), ("fixed_scale", C.c_float),
+        ("rho", C.c_float), ("flip_blend", C.c_float), ("wall_friction", C.c_float),
+        ("origin", C.c_float * 3), ("grav", C.c_float * 3),
+        ("rest", C.c_float), ("stiff", C.c_float), ("drift_clamp", C.c_float),
+        ("liquid_threshold", C.c_float), ("surface_dilation", C.c_float), ("cfl", C.c_float),
+        ("tol_sq", C.c_float), ("tol", C.c_float),
+    ]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [
+        ("liquid_cells", C.c_uint32), ("cg_iterations", C.c_uint32),
+        ("misc", C.c_uint32), ("pad", C.c_uint32),
+        ("rs_initial", C.c_double), ("rs_final", C.c_double), ("true_residual_sq", C.c_double),
+    ]
+
+
+def _outputs_type(real, acc):
+    rp = C.POINTER(real)
+    ap = C.POINTER(acc)
+
+    class Out(C.Structure):
+        _fields_ = [
+            ("occupancy", _u32p), ("cell_offset", _u32p), ("perm", _u32p), ("cell_type", _u32p),
+            ("num", ap * 3), ("den", ap * 3),
+            ("vel", rp * 3), ("saved", rp * 3), ("vel_final", rp * 3),
+            ("divergence", rp), ("stats", C.POINTER(OrcStats)),
+        ]
+    return Out
+
+
+OutF32 = _outputs_type(C.c_float, C.c_int32)
+OutF64 = _outputs_type(C.c_double, C.c_int64)
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_cell_key_f32.restype = C.c_uint32
+        _lib.orc_cell_key_f32.argtypes = [C.POINTER(OrcParams), C.c_float, C.c_float, C.c_float]
+        _lib.orc_classify_u32.restype = C.c_uint32
+        _lib.orc_pressure_solve_f32.restype = C.c_uint32
+        _lib.orc_pressure_solve_f64.restype = C.c_uint32
+        _lib.orc_run_f32.restype = C.c_int
+        _lib.orc_run_f64.restype = C.c_int
+        _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_sample_axis_f32.argtypes = [
+            C.POINTER(OrcParams), C.c_int, C.c_float, C.c_float, C.c_float,
+            C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _typed(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_params(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), dt=1.0 / 120.0, rho=1000.0,
+                flip_blend=0.95, wall_friction=0.0, grav=(0.0, -9.81, 0.0), rest=8.0,
+                stiff=0.45, drift_clamp=0.5, liquid_threshold=1, surface_dilation=0, cfl=2.0,
+                pressure_iters=200, warm_start=1, tol=1e-5, reduce_mode=0,
+                fixed_scale=65536.0, clamp_tol=True) -> OrcParams:
+    """Params uniform as src/gpu/fluid.rs:289-318 fills it (inv_h = f32(1/h))."""
+    P = OrcParams()
+    P.nx, P.ny, P.nz = nx, ny, nz
+    P.pressure_iters = max(int(pressure_iters), 1)
+    P.warm_start = int(bool(warm_start))
+    P.reduce_mode = int(reduce_mode)
+    h32 = np.float32(h)
+    P.h = float(h32)
+    P.inv_h = float(np.float32(1.0) / h32)
+    P.dt = float(np.float32(dt))
+    P.fixed_scale = fixed_scale
+    P.rho = rho
+    P.flip_blend = flip_blend
+    P.wall_friction = wall_friction
+    for a in range(3):
+        P.origin[a] = float(np.float32(origin[a]))
+        P.grav[a] = float(np.float32(grav[a]))
+    P.rest, P.stiff, P.drift_clamp = rest, stiff, drift_clamp
+    P.liquid_threshold = float(liquid_threshold)
+    P.surface_dilation = float(surface_dilation)
+    P.cfl = cfl
+    t = np.float32(tol)
+    if clamp_tol:  # src/gpu/fluid.rs:47-50
+        t = np.float32(min(max(float(t), 0.0), 0.1))
+    P.tol = float(t)
+    P.tol_sq = float(t * t)
+    return P
+
+
+def face_count(P, a):
+    d = [P.nx, P.ny, P.nz]
+    d[a] += 1
+    return d[0] * d[1] * d[2]
+
+
+def cell_key(P, x, y, z) -> int:
+    return int(lib().orc_cell_key_f32(C.byref(P), x, y, z))
+
+
+def exclusive_scan(hist) -> np.ndarray:
+    hist = np.ascontiguousarray(hist, dtype=np.uint32)
+    out = np.zeros_like(hist)
+    lib().orc_exclusive_scan_u32(C.c_uint32(hist.size), ptr(hist), ptr(out))
+    return out
+
+
+def stable_permutation(keys, cells) -> np.ndarray:
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    perm = np.zeros(keys.size, dtype=np.uint32)
+    lib().orc_stable_permutation(C.c_uint32(keys.size), C.c_uint32(cells), ptr(keys), ptr(perm))
+    return perm
+
+
+def histogram(P, pos) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    px, py, pz = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    occ = np.zeros(P.nx * P.ny * P.nz, dtype=np.uint32)
+    lib().orc_histogram_f32(C.byref(P), C.c_uint32(len(px)), ptr(px), ptr(py), ptr(pz), ptr(occ))
+    return occ
+
+
+def classify(P, occ):
+    occ = np.ascontiguousarray(occ, dtype=np.uint32)
+    ct = np.zeros_like(occ)
+    n = lib().orc_classify_u32(C.byref(P), ptr(occ), ptr(ct))
+    return ct, int(n)
+
+
+def p2g(P, pos, vel):
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    cols = [np.ascontiguousarray(pos[:, a]) for a in range(3)] + \
+           [np.ascontiguousarray(vel[:, a]) for a in range(3)]
+    num = [np.zeros(face_count(P, a), np.int32) for a in range(3)]
+    den = [np.zeros(face_count(P, a), np.int32) for a in range(3)]
+    nump = (C.c_void_p * 3)(*[ptr(x) for x in num])
+    denp = (C.c_void_p * 3)(*[ptr(x) for x in den])
+    lib().orc_p2g_f32(C.byref(P), C.c_uint32(len(pos)), *[ptr(c) for c in cols], nump, denp)
+    return num, den
+
+
+def divergence(P, vel, ct, occ=None):
+    vel = [np.ascontiguousarray(v, dtype=np.float32) for v in vel]
+    ct = np.ascontiguousarray(ct, dtype=np.uint32)
+    if occ is None:
+        occ = np.zeros(ct.size, np.uint32)
+    occ = np.ascontiguousarray(occ, dtype=np.uint32)
+    div = np.zeros(ct.size, np.float32)
+    vp = (C.c_void_p * 3)(*[ptr(v) for v in vel])
+    lib().orc_divergence_f32(C.byref(P), vp, ptr(ct), ptr(occ), ptr(div))
+    return div
+
+
+def apply_poisson(P, ct, x, f64=False):
+    ct = np.ascontiguousarray(ct, dtype=np.uint32)
+    dt = np.float64 if f64 else np.float32
+    x = np.ascontiguousarray(x, dtype=dt)
+    out = np.zeros_like(x)
+    fn = lib().orc_apply_poisson_f64 if f64 else lib().orc_apply_poisson_f32
+    fn(C.byref(P), ptr(ct), ptr(x), ptr(out))
+    return out
+
+
+def pressure_solve(P, ct, div, p0=None, f64=False):
+    """Returns (pressure, OrcStats).  p0 is the warm-start state (pressure_a)."""
+    ct = np.ascontiguousarray(ct, dtype=np.uint32)
+    dt = np.float64 if f64 else np.float32
+    div = np.ascontiguousarray(div, dtype=dt)
+    p = np.zeros(ct.size, dt) if p0 is None else np.array(p0, dtype=dt, copy=True)
+    st = OrcStats()
+    fn = lib().orc_pressure_solve_f64 if f64 else lib().orc_pressure_solve_f32
+    fn(C.byref(P), ptr(ct), ptr(div), ptr(p), C.byref(st))
+    return p, st
+
+
+def jacobi_solve(P, ct, div, iters):
+    ct = np.ascontiguousarray(ct, dtype=np.uint32)
+    div = np.ascontiguousarray(div, dtype=np.float32)
+    p = np.zeros(ct.size, np.float32)
+    lib().orc_jacobi_solve_f32(C.byref(P), ptr(ct), ptr(div), C.c_uint32(iters), ptr(p))
+    return p
+
+
+def gradient_enforce(P, p, ct, vel):
+    vel = [np.array(v, dtype=np.float32, copy=True) for v in vel]
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    ct = np.ascontiguousarray(ct, dtype=np.uint32)
+    vp = (C.c_void_p * 3)(*[ptr(v) for v in vel])
+    lib().orc_gradient_enforce_f32(C.byref(P), ptr(p), ptr(ct), vp)
+    return vel
+
+
+def sample_axis(P, axis, point, vel, saved, wall_aware=True):
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    saved = np.ascontiguousarray(saved, dtype=np.float32)
+    f = C.c_float()
+    s = C.c_float()
+    lib().orc_sample_axis_f32(C.byref(P), axis, point[0], point[1], point[2], ptr(vel), ptr(saved),
+                              int(wall_aware), C.byref(f), C.byref(s))
+    return f.value, s.value
+
+
+def g2p(P, pos, pvel, vel, saved):
+    pos = np.array(pos, dtype=np.float32, copy=True)
+    pvel = np.array(pvel, dtype=np.float32, copy=True)
+    cols = [np.ascontiguousarray(pos[:, a]) for a in range(3)] + \
+           [np.ascontiguousarray(pvel[:, a]) for a in range(3)]
+    vel = [np.ascontiguousarray(v, dtype=np.float32) for v in vel]
+    saved = [np.ascontiguousarray(v, dtype=np.float32) for v in saved]
+    vp = (C.c_void_p * 3)(*[ptr(v) for v in vel])
+    sp = (C.c_void_p * 3)(*[ptr(v) for v in saved])
+    lib().orc_g2p_f32(C.byref(P), C.c_uint32(len(pos)), *[ptr(c) for c in cols], vp, sp)
+    return np.stack(cols[:3], 1), np.stack(cols[3:], 1)
+
+
+class StepResult:
+    pass
+
+
+def run(P, pos, vel, pressure=None, steps=1, sort_period=1, tick0=0, f64=False, want=True):
+    """Run `steps` substeps of the restated reference step from (pos, vel,
+    pressure); returns a StepResult with the final particles and the last
+    substep's grid outputs."""
+    real = np.float64 if f64 else np.float32
+    acc = np.int64 if f64 else np.int32
+    n = len(pos)
+    Cc = P.nx * P.ny * P.nz
+    cols = [np.array(pos[:, a], dtype=real, copy=True) for a in range(3)] + \
+           [np.array(vel[:, a], dtype=real, copy=True) for a in range(3)]
+    pr = np.zeros(Cc, real) if pressure is None else np.array(pressure, dtype=real, copy=True)
+    res = StepResult()
+    out = (OutF64 if f64 else OutF32)()
+    st = OrcStats()
+    if want:
+        res.occupancy = np.zeros(Cc, np.uint32)
+        res.cell_offset = np.zeros(Cc, np.uint32)
+        res.perm = np.zeros(max(n, 1), np.uint32)
+        res.cell_type = np.zeros(Cc, np.uint32)
+        res.divergence = np.zeros(Cc, real)
+        res.num = [np.zeros(face_count(P, a), acc) for a in range(3)]
+        res.den = [np.zeros(face_count(P, a), acc) for a in range(3)]
+        res.vel = [np.zeros(face_count(P, a), real) for a in range(3)]
+        res.saved = [np.zeros(face_count(P, a), real) for a in range(3)]
+        res.vel_final = [np.zeros(face_count(P, a), real) for a in range(3)]
+        ct_real = C.c_double if f64 else C.c_float
+        ct_acc = C.c_int64 if f64 else C.c_int32
+        out.occupancy = _typed(res.occupancy, C.c_uint32)
+        out.cell_offset = _typed(res.cell_offset, C.c_uint32)
+        out.perm = _typed(res.perm, C.c_uint32)
+        out.cell_type = _typed(res.cell_type, C.c_uint32)
+        out.divergence = _typed(res.divergence, ct_real)
+        for a in range(3):
+            out.num[a] = _typed(res.num[a], ct_acc)
+            out.den[a] = _typed(res.den[a], ct_acc)
+            out.vel[a] = _typed(res.vel[a], ct_real)
+            out.saved[a] = _typed(res.saved[a], ct_real)
+            out.vel_final[a] = _typed(res.vel_final[a], ct_real)
+    out.stats = C.pointer(st)
+    fn = lib().orc_run_f64 if f64 else lib().orc_run_f32
+    res.sorted = bool(fn(C.byref(P), C.c_uint32(n), *[ptr(c) for c in cols], ptr(pr),
+                         C.c_uint32(steps), C.c_uint32(sort_period), C.c_uint32(tick0),
+                         C.byref(out)))
+    res.pos = np.stack(cols[:3], 1)
+    res.vel_particles = np.stack(cols[3:], 1)
+    res.pressure = pr
+    res.stats = st
+    if want:
+        res.perm = res.perm[:n]
+    return res
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(C.c_int(n))

@@ -1,0 +1,265 @@
+"""Parity of the HIP step (through the C-ABI) against the CPU restatement.
+
+Bar: bit-exact for binning (occupancy, bin starts, permutation) and for every
+integer/P2G result; within 1e-5 relative L2 for face velocities, pressure and
+particles per step, starting both sides from the same state (teacher-forced).
+The reference-structured engine mirrors the f32 WGSL reduction trees and is
+additionally expected to match bit for bit; the fused engine computes its CG
+dots in f64, like src/sim/pressure.rs:182-188.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fluidstep as fsm
+import oracle
+import scenes
+from helpers import bits_equal, gpu_params, oracle_params, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = [fsm.ENGINE_REFERENCE, fsm.ENGINE_FUSED]
+TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def reduce_mode(engine):
+    return 0 if engine == fsm.ENGINE_REFERENCE else 1
+
+
+def run_pair(scene, engine, steps=1, pos=None, vel=None, pressure=None, velocity=fsm.VELOCITY_F32, **over):
+    """One teacher-forced comparison: both sides start from (pos, vel,
+    pressure) and run `steps` substeps (sort every substep)."""
+    pos = scene["pos"] if pos is None else pos
+    vel = scene["vel"] if vel is None else vel
+    P = oracle_params(scene, reduce_mode(engine), **over)
+    ref = oracle.run(P, pos, vel, pressure=pressure, steps=steps, sort_period=1)
+    t = fsm.FluidStep(gpu_params(scene, engine, velocity=velocity, **over), pos, vel)
+    if pressure is not None:
+        t.load_state(pos, vel, pressure)
+    t.step(steps)
+    t.synchronize()
+    return ref, t
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_binning_bit_exact(engine):
+    sc = scenes.dam_break_32()
+    say("binning", sc["name"], "engine", engine, "N", len(sc["pos"]))
+    ref, t = run_pair(sc, engine)
+    assert np.array_equal(t.occupancy(), ref.occupancy)
+    assert np.array_equal(t.cell_offset(), ref.cell_offset)
+    assert np.array_equal(t.permutation(), ref.perm)
+    assert t.stats()["sorted"] == 1
+    t.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_step_parity_dam_break(engine):
+    sc = scenes.dam_break_32()
+    say("step parity", sc["name"], "engine", engine)
+    ref, t = run_pair(sc, engine)
+    st = t.stats()
+    assert np.array_equal(t.cell_type(), ref.cell_type)
+    assert st["liquid_cells"] == ref.stats.liquid_cells
+    saved = t.faces(fsm.FACE_SAVED)
+    vel = t.faces(fsm.FACE_VELOCITY)
+    for a in range(3):
+        assert bits_equal(saved[a], ref.saved[a]), f"saved axis {a}"
+        assert rel_l2(vel[a], ref.vel_final[a]) <= TOL
+    assert rel_l2(t.pressure(), ref.pressure) <= TOL
+    pos, pv = t.particles()
+    assert rel_l2(pos, ref.pos) <= TOL
+    assert rel_l2(pv, ref.vel_particles) <= TOL
+    if engine == fsm.ENGINE_REFERENCE:
+        assert st["cg_iterations"] == ref.stats.cg_iterations
+        assert bits_equal(t.pressure(), ref.pressure)
+        assert bits_equal(pos, ref.pos) and bits_equal(pv, ref.vel_particles)
+    else:
+        assert st["p2g_gather"] == 1
+        assert abs(st["cg_iterations"] - ref.stats.cg_iterations) <= 1
+    t.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_teacher_forced_steps(engine):
+    sc = scenes.small_tank(n=16, seed=7)
+    P = oracle_params(sc, reduce_mode(engine))
+    pos, vel, pr = sc["pos"], sc["vel"], np.zeros(16 ** 3, np.float32)
+    t = fsm.FluidStep(gpu_params(sc, engine), pos, vel)
+    for s in range(4):
+        say("teacher-forced", sc["name"], "engine", engine, "step", s)
+        ref = oracle.run(P, pos, vel, pressure=pr, steps=1, sort_period=1)
+        t.load_state(pos, vel, pr)
+        t.step(1)
+        gp, gv = t.particles()
+        assert rel_l2(t.pressure(), ref.pressure) <= TOL
+        assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+        for a, f in enumerate(t.faces()):
+            assert rel_l2(f, ref.vel_final[a]) <= TOL
+        pos, vel, pr = ref.pos, ref.vel_particles, ref.pressure
+    t.close()
+
+
+def _misfit_coordinate(n):
+    """A float32 x in (0,1) where floor(x/h) != floor(x*f32(1/h)) for h=f32(1/n)."""
+    h = np.float32(1.0) / np.float32(n)
+    inv = np.float32(1.0) / h
+    for c in range(2, n - 2):
+        x0 = np.float32(c) * h
+        x = x0
+        for _ in range(64):
+            if np.floor(x / h) != np.floor(x * inv):
+                return float(x)
+            x = np.nextafter(x, np.float32(0))
+        x = x0
+        for _ in range(64):
+            if np.floor(x / h) != np.floor(x * inv):
+                return float(x)
+            x = np.nextafter(x, np.float32(1))
+    return None
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_non_power_of_two_grid_with_misfit(engine):
+    """24^3 (h = 1/24): a particle where the sort cell (divide by h) and the
+    P2G cell (multiply by 1/h) differ forces the fused engine onto its atomic
+    P2G fallback; results still match."""
+    sc = scenes.small_tank(n=24, seed=11, fill=0.45)
+    x = _misfit_coordinate(24)
+    assert x is not None
+    pos = sc["pos"].copy()
+    pos[5, 0] = x
+    say("misfit", sc["name"], "engine", engine, "x", x)
+    ref, t = run_pair(sc, engine, pos=pos)
+    st = t.stats()
+    if engine == fsm.ENGINE_FUSED:
+        assert st["misfits"] >= 1 and st["p2g_gather"] == 0
+    assert np.array_equal(t.permutation(), ref.perm)
+    saved = t.faces(fsm.FACE_SAVED)
+    for a in range(3):
+        assert bits_equal(saved[a], ref.saved[a])
+    assert rel_l2(t.pressure(), ref.pressure) <= TOL
+    gp, gv = t.particles()
+    assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+    t.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_escaped_particles_clamp_into_edge_cells(engine):
+    """Particles outside the tank bin into the clamped edge cells (src/lib.rs:
+    1286-1300) and G2P pulls them back inside."""
+    sc = scenes.small_tank(n=16, seed=3)
+    pos = sc["pos"].copy()
+    pos[0] = (-0.3, 0.5, 0.5)
+    pos[1] = (1.7, 1.2, -0.1)
+    say("escaped", engine)
+    ref, t = run_pair(sc, engine, pos=pos)
+    assert np.array_equal(t.occupancy(), ref.occupancy)
+    assert np.array_equal(t.permutation(), ref.perm)
+    gp, gv = t.particles()
+    assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+    t.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_empty_tank(engine):
+    sc = scenes.small_tank(n=16)
+    t = fsm.FluidStep(gpu_params(sc, engine), np.zeros((0, 3), np.float32))
+    say("empty", engine)
+    t.step(2)
+    st = t.stats()
+    assert st["liquid_cells"] == 0
+    assert np.all(t.pressure() == 0)
+    t.close()
+
+
+def test_fused_is_deterministic():
+    sc = scenes.dam_break_32()
+    say("determinism")
+    outs = []
+    for _ in range(2):
+        t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED), sc["pos"], sc["vel"])
+        t.step(5)
+        pos, vel = t.particles()
+        outs.append((pos, vel, t.pressure()))
+        t.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert bits_equal(a, b)
+
+
+def test_bf16_velocity_storage():
+    """BF16 particle velocities: P2G reads bf16, G2P stores bf16 (f32 math).
+    Against the oracle fed the same bf16-rounded velocities: positions match
+    to 1e-5 and the stored velocity is the bf16 rounding of the f32 result."""
+    sc = scenes.dam_break_32()
+    rng = np.random.default_rng(0)
+    vel = (rng.random(sc["pos"].shape, dtype=np.float32) - 0.5).astype(np.float32)
+    vb = (vel.view(np.uint32) + 0x7FFF + ((vel.view(np.uint32) >> 16) & 1)) & 0xFFFF0000
+    vel_b = vb.astype(np.uint32).view(np.float32)
+    say("bf16")
+    ref, t = run_pair(sc, fsm.ENGINE_FUSED, vel=vel_b, velocity=fsm.VELOCITY_BF16)
+    gp, gv = t.particles()
+    assert rel_l2(gp, ref.pos) <= TOL
+    rv = ref.vel_particles
+    rb = ((rv.view(np.uint32) + 0x7FFF + ((rv.view(np.uint32) >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    assert rel_l2(gv, rb.view(np.float32)) <= 1e-4
+    t.close()
+
+
+def test_live_setters_change_the_step():
+    sc = scenes.small_tank(n=16, seed=9)
+    t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED), sc["pos"], sc["vel"])
+    r = t.set_param("physics.flip_blend", 0.5)
+    assert r["status"] == "applied" and r["applied_live"]
+    r = t.set_param("physics.cfl", 1e100)
+    assert r["stored_value"] == 6.0 and r["clamped"]
+    say("setters")
+    ref = oracle.run(oracle_params(sc, 1, flip_blend=0.5, cfl=6.0), sc["pos"], sc["vel"], steps=1)
+    t.step(1)
+    gp, gv = t.particles()
+    assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+    t.close()
+
+
+def test_impulse_and_reset():
+    sc = scenes.small_tank(n=16, seed=2)
+    t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED), sc["pos"], sc["vel"])
+    t.apply_impulse(0.25, -0.5, 1.0)
+    _, v = t.particles()
+    exp = sc["vel"] + np.array([0.25, -0.5, 1.0], np.float32)
+    assert bits_equal(v, exp.astype(np.float32))
+    t.step(2)
+    t.reset()
+    p, v = t.particles()
+    assert bits_equal(p, sc["pos"]) and bits_equal(v, sc["vel"])
+    assert np.all(t.pressure() == 0)
+    t.close()
+
+
+def test_full_size_high_density_one_step():
+    """128^3, ~6.7M particles: one teacher-forced step of the fused engine
+    against the restatement (f64 dots), plus size-independent invariants."""
+    sc = scenes.high_density_128()
+    N = len(sc["pos"])
+    say("full size", sc["name"], "N", N)
+    ref, t = run_pair(sc, fsm.ENGINE_FUSED)
+    occ = t.occupancy()
+    assert int(occ.sum()) == N
+    off = t.cell_offset()
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(occ, dtype=np.uint64)[:-1]]).astype(np.uint32))
+    perm = t.permutation()
+    assert np.array_equal(perm, ref.perm)
+    st = t.stats()
+    assert st["cg_iterations"] < 200
+    assert st["rs_final"] <= (1e-5 ** 2) * st["rs_initial"] * 1.0001
+    assert rel_l2(t.pressure(), ref.pressure) <= TOL
+    gp, gv = t.particles()
+    assert np.all(np.isfinite(gp)) and np.all(np.isfinite(gv))
+    assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+    t.close()
