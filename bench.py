@@ -82,6 +82,79 @@ def cpu_baseline(sc, steps):
     return dt, res, oracle.num_threads()
 
 
+def _track(ids, perm):
+    """Compose the sort permutation into the particle-identity array."""
+    return ids[perm] if perm is not None else ids
+
+
+def precision_report(steps, teacher_states):
+    """Error of each GPU precision mode against the FP64 restatement on the
+    128^3 workload: (a) free-running for `steps` steps (particles compared by
+    identity through the sort permutations), (b) one step from the FP64
+    oracle's own state, averaged over `teacher_states` states."""
+    import fluidstep as fsm
+    import oracle
+    import scenes
+    from helpers import gpu_params, oracle_params, rel_l2
+
+    sc = scenes.high_density_128()
+    N = len(sc["pos"])
+    P64 = oracle_params(sc, 1)
+    modes = {
+        "mixed_f32_velocity": gpu_params(sc, fsm.ENGINE_FUSED),
+        "mixed_bf16_velocity": gpu_params(sc, fsm.ENGINE_FUSED, velocity=fsm.VELOCITY_BF16),
+        "plain_fp32": gpu_params(sc, fsm.ENGINE_REFERENCE),
+    }
+    # FP64 oracle trajectory, keeping a few states for the one-step errors.
+    pos, vel = sc["pos"].astype(np.float64), sc["vel"].astype(np.float64)
+    pr = np.zeros(sc["n"] ** 3)
+    ids64 = np.arange(N)
+    states = []
+    keep = set(np.linspace(0, steps - 1, teacher_states).astype(int).tolist()) if teacher_states else set()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        if s in keep:
+            states.append((pos.copy(), vel.copy(), pr.copy()))
+        r = oracle.run(P64, pos, vel, pressure=pr, steps=1, sort_period=1, f64=True)
+        ids64 = _track(ids64, r.perm)
+        pos, vel, pr = r.pos, r.vel_particles, r.pressure
+        print(f"fp64 oracle step {s + 1}/{steps} cg={r.stats.cg_iterations}", file=sys.stderr, flush=True)
+    t_cpu = time.perf_counter() - t0
+    order64 = np.argsort(ids64)
+    ref_pos, ref_vel, ref_pr = pos[order64], vel[order64], pr
+    report = {"workload": "high_density_128", "steps": steps, "particles": N,
+              "fp64_oracle_seconds": t_cpu, "modes": {}}
+    for name, prm in modes.items():
+        t = fsm.FluidStep(prm, sc["pos"], sc["vel"])
+        ids = np.arange(N)
+        for s in range(steps):
+            t.step(1)
+            ids = _track(ids, t.permutation().astype(np.int64))
+        gp, gv = t.particles()
+        order = np.argsort(ids)
+        ent = {
+            "free_run_pos_rel_l2": rel_l2(gp[order], ref_pos),
+            "free_run_vel_rel_l2": rel_l2(gv[order], ref_vel),
+            "free_run_pressure_rel_l2": rel_l2(t.pressure(), ref_pr),
+        }
+        one = {"pos": [], "vel": [], "pressure": []}
+        for (sp, sv, spr) in states:
+            r = oracle.run(P64, sp, sv, pressure=spr, steps=1, sort_period=1, f64=True)
+            t.load_state(sp.astype(np.float32), sv.astype(np.float32), spr.astype(np.float32))
+            t.step(1)
+            gp, gv = t.particles()
+            one["pos"].append(rel_l2(gp, r.pos))
+            one["vel"].append(rel_l2(gv, r.vel_particles))
+            one["pressure"].append(rel_l2(t.pressure(), r.pressure))
+        for k, v in one.items():
+            if v:
+                ent[f"one_step_{k}_rel_l2_max"] = float(np.max(v))
+        t.close()
+        report["modes"][name] = ent
+        print(name, json.dumps(ent), file=sys.stderr, flush=True)
+    return report
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +165,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--bf16", action="store_true", help="bf16 particle-velocity storage")
+    ap.add_argument("--precision-report", type=int, default=0, metavar="STEPS",
+                    help="instead of timing, report each GPU precision mode's error against the FP64 "
+                         "restatement after STEPS steps (and one-step errors from its states)")
+    ap.add_argument("--teacher-states", type=int, default=3)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -104,6 +181,10 @@ def main():
     import fluidstep as fsm
     import scenes
     from helpers import gpu_params
+
+    if args.precision_report:
+        print(json.dumps(precision_report(args.precision_report, args.teacher_states)), flush=True)
+        return
 
     sc = scenes.high_density_128()
     N = len(sc["pos"])

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -576,9 +577,20 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         const size_t ntiles = C / 64 + 1;
         t->cg_blocks = cg_persistent_max_blocks();
         if (t->cg_blocks <= 0) return fail(FS_ERR_DEVICE, "cannot size the persistent CG grid");
-        // Fill the chip with whole XCD groups (multiple of 8 blocks).
-        t->cg_blocks = (t->cg_blocks / 8) * 8;
+        const int max_blocks = t->cg_blocks;
+        // The occupancy figure is not a residency guarantee: on MI355X a grid
+        // of the full figure (8 blocks of 256 per CU) did not become resident
+        // at once and its grid barrier timed out, while half of it did.  Use
+        // half, in whole XCD groups (multiple of 8 blocks).
+        t->cg_blocks = (t->cg_blocks / 2 / 8) * 8;
+        if (const char* env = std::getenv("FLUIDSTEP_CG_BLOCKS")) {
+            int v = std::atoi(env);
+            if (v > 0 && v <= max_blocks) t->cg_blocks = v;
+        }
         if (t->cg_blocks <= 0) t->cg_blocks = 1;
+        if (std::getenv("FLUIDSTEP_VERBOSE"))
+            std::fprintf(stderr, "fluidstep: persistent CG grid %d blocks (co-resident max %d)\n", t->cg_blocks,
+                         max_blocks);
         FS_HIP(t->alloc(t->cg.bar_count, 1));
         FS_HIP(t->alloc(t->cg.bar_gen, 1));
         FS_HIP(t->alloc(t->cg.abort_flag, 1));
