@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -166,7 +167,11 @@ struct fs_tank {
     float* partials = nullptr;  // reference CG partials
     uint32_t* counters = nullptr;  // [0] liquid cells, [1] misfits, [2] CG iterations (reference engine)
     CgWork cg{};
-    int cg_blocks = 0;
+    int cg_blocks = 0;        // persistent form
+    int cg_mode = 1;          // 0 persistent kernel, 1 two kernels per iteration
+    int cg_multi_blocks = 0;  // multi-kernel form
+    int cg_use_graph = 1;     // replay the multi-kernel solve from a captured hipGraph
+    hipGraphExec_t cg_exec = nullptr;
 
     uint64_t tick = 0;  // sort tick (advance_sort_tick, src/gpu/fluid.rs:1491-1498)
     uint64_t substeps = 0;
@@ -190,7 +195,12 @@ struct fs_tank {
         ptr = (T*)p_;
         return hipSuccess;
     }
+    void drop_graph() {
+        if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
+        cg_exec = nullptr;
+    }
     ~fs_tank() {
+        drop_graph();
         for (void* a : allocs) (void)hipFree(a);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -375,9 +385,33 @@ int substep_fused(fs_tank* t, bool sort_now) {
     if (t->timing) FS_HIP(hipEventRecord(t->ev[1], s));
     FS_HIP(launch_div_cginit(P, t->vel, t->ct, t->occ, t->div, t->b, t->p, t->r, t->d0, t->cg, t->scan_ws, s));
     FS_HIP(launch_fill_u32(t->cg.bar_count, 0u, 1, s));
+    FS_HIP(launch_fill_u32(t->cg.group_count, 0u, 8 * 32, s));
     FS_HIP(launch_fill_u32(t->cg.abort_flag, 0u, 1, s));
     if (t->timing) FS_HIP(hipEventRecord(t->ev[4], s));
-    FS_HIP(launch_cg_persistent(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_blocks, s));
+    if (t->cg_mode == 0) {
+        FS_HIP(launch_cg_persistent(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_blocks, s));
+    } else if (!t->cg_use_graph) {
+        FS_HIP(launch_cg_multi(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_multi_blocks, s));
+    } else {
+        if (!t->cg_exec) {
+            // Capture the whole solve (2 x cap + 2 kernels) once; the graph is
+            // dropped whenever the step constants change.
+            hipGraph_t g = nullptr;
+            FS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t le =
+                launch_cg_multi(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_multi_blocks, s);
+            hipError_t ce = hipStreamEndCapture(s, &g);
+            if (le != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(FS_ERR_DEVICE, std::string("CG capture: ") + hipGetErrorString(le));
+            }
+            FS_HIP(ce);
+            hipError_t ie = hipGraphInstantiate(&t->cg_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            FS_HIP(ie);
+        }
+        FS_HIP(hipGraphLaunch(t->cg_exec, s));
+    }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[2], s));
     FS_HIP(launch_gradient_enforce(P, t->p, t->ct, t->vel, s));
     FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
@@ -588,13 +622,26 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
             if (v > 0 && v <= max_blocks) t->cg_blocks = v;
         }
         if (t->cg_blocks <= 0) t->cg_blocks = 1;
+        int dev = 0, cus = 0;
+        FS_HIP(hipGetDevice(&dev));
+        FS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        t->cg_multi_blocks = 4 * cus;
+        if (const char* env = std::getenv("FLUIDSTEP_CG_MULTI_BLOCKS")) {
+            int v = std::atoi(env);
+            if (v > 0 && v <= 65536) t->cg_multi_blocks = v;
+        }
+        if (const char* env = std::getenv("FLUIDSTEP_CG_MODE")) t->cg_mode = std::strcmp(env, "persistent") == 0 ? 0 : 1;
+        if (const char* env = std::getenv("FLUIDSTEP_CG_GRAPH")) t->cg_use_graph = std::atoi(env) != 0;
         if (std::getenv("FLUIDSTEP_VERBOSE"))
-            std::fprintf(stderr, "fluidstep: persistent CG grid %d blocks (co-resident max %d)\n", t->cg_blocks,
-                         max_blocks);
+            std::fprintf(stderr, "fluidstep: CG %s, persistent grid %d (co-resident max %d), multi grid %d, graph %d\n",
+                         t->cg_mode == 0 ? "persistent" : "multi-kernel", t->cg_blocks, max_blocks,
+                         t->cg_multi_blocks, t->cg_use_graph);
         FS_HIP(t->alloc(t->cg.bar_count, 1));
+        FS_HIP(t->alloc(t->cg.group_count, 8 * 32));
+        FS_HIP(t->alloc(t->cg.nmask, C));
         FS_HIP(t->alloc(t->cg.bar_gen, 1));
         FS_HIP(t->alloc(t->cg.abort_flag, 1));
-        FS_HIP(t->alloc(t->cg.partials, (size_t)t->cg_blocks));
+        FS_HIP(t->alloc(t->cg.partials, (size_t)std::max(t->cg_blocks, t->cg_multi_blocks)));
         FS_HIP(t->alloc(t->cg.result, 4));
         FS_HIP(t->alloc(t->cg.tile_flags, ntiles));
         FS_HIP(t->alloc(t->cg.tile_scan, ntiles));
@@ -731,6 +778,7 @@ FS_API int fs_set_param(fs_tank* t, const char* id, double value, fs_mutation* o
         if (changed) {
             FS_HIP(hipStreamSynchronize(t->stream));
             build_step_params(q, t->N, t->P);
+            t->drop_graph();
             FS_HIP(hipMemcpyAsync(t->scal + 6, &t->P.tol_sq, sizeof(float), hipMemcpyHostToDevice, t->stream));
             FS_HIP(hipStreamSynchronize(t->stream));
         }
@@ -744,7 +792,9 @@ FS_API int fs_set_gravity(fs_tank* t, float gx, float gy, float gz) {
     if (int rc = check_tank(t)) return rc;
     if (!std::isfinite(gx) || !std::isfinite(gy) || !std::isfinite(gz)) return fail(FS_ERR_NOT_FINITE, "non-finite gravity");
     t->prm.gravity[0] = gx; t->prm.gravity[1] = gy; t->prm.gravity[2] = gz;
+    FS_HIP(hipStreamSynchronize(t->stream));
     build_step_params(t->prm, t->N, t->P);
+    t->drop_graph();
     return FS_OK;
 }
 

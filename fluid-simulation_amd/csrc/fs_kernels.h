@@ -93,6 +93,7 @@ hipError_t launch_cg_iter_count_ref(const float* scal, uint32_t* count, hipStrea
 struct CgState {
     double rs_old, rs_init, dq, rr;
     double true_res_sq;
+    float alpha, beta;       // f32 step lengths of the current iteration
     uint32_t iterations;
     uint32_t active;
     uint32_t error;          // nonzero: a grid barrier timed out
@@ -101,7 +102,9 @@ struct CgState {
 struct CgWork {
     // grid-barrier + reduction scratch
     uint32_t* bar_count;     // [1]
+    uint32_t* group_count;   // [8 * 32]: one arrival counter per 128-byte line
     uint32_t* bar_gen;       // [1]
+    uint16_t* nmask;         // [C] stencil mask of liquid cells
     uint32_t* abort_flag;    // [1]
     double* partials;        // [max_blocks]
     double* result;          // [4]
@@ -118,6 +121,11 @@ hipError_t launch_cg_persistent(const StepParams& P, const uint8_t* ct, const fl
                                 float* r, float* d0, float* d1, float* q, const CgWork& w,
                                 int grid_blocks, hipStream_t s);
 int cg_persistent_max_blocks();
+// Same solve as two kernels per iteration (SpMV+d.q, update+r.r) with
+// last-block-done reductions; P.iters iterations are enqueued (converged
+// iterations return at once).  Suitable for hipGraph capture.
+hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* b, float* p, float* r,
+                           float* d0, float* d1, float* q, const CgWork& w, int grid_blocks, hipStream_t s);
 
 // ---- k_g2p.hip ----
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16,

@@ -1,0 +1,603 @@
+// k_cg_fused.hip — the mixed-precision pressure solve of the fused engine.
+//
+//  * k_div_cginit: divergence (sh/divergence.wgsl) and CG init
+//    (sh/cg_init.wgsl) per cell in one pass; it also packs each liquid cell's
+//    stencil into a 16-bit mask (non-solid neighbour count, which neighbours
+//    are liquid) and flags the 64-cell tiles that hold liquid.
+//  * the liquid tiles are compacted into a list (exclusive scan), so the solve
+//    never touches all-air or all-solid tiles.
+//  * the solve itself, in one of two forms:
+//      - k_cg_persistent: one launch for the whole solve, two grid-wide
+//        reductions per iteration;
+//      - k_cg_phase_a / k_cg_phase_b: two launches per iteration, each ending
+//        in a last-block-done reduction (no block ever waits), enqueued for
+//        the iteration cap and replayed from a hipGraph by the host.
+//    Per iteration: phase A forms dn = r + beta d at every stencil point
+//    (identical expression, so neighbours agree bit for bit with the
+//    reference's separate sh/cg_dir.wgsl pass), q = A dn and dn.q; phase B
+//    updates p += alpha dn, r -= alpha q and r.r.  Vectors are f32; dots are
+//    f64 summed in a fixed order; alpha and beta are rounded to f32 as
+//    src/sim/pressure.rs:282,296 do; the stop test rs_new <= tol^2 rs_init is
+//    f64 (src/sim/pressure.rs:262-292).
+//  * the f64 true residual |b - A p|^2 is reduced after the loop.
+#include "fs_kernels.h"
+#include "fs_face.h"
+
+namespace fs {
+
+namespace {
+inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+constexpr int kTile = 64;             // cells per CG tile (one wave)
+constexpr int kCgThreads = 512;       // persistent workgroup (8 waves)
+constexpr int kCgMultiThreads = 256;  // multi-kernel workgroup
+constexpr int kU = 4;                 // tiles a wave loads before computing (memory-level parallelism)
+constexpr int kLdsTiles = 2048;       // tile ids a persistent block keeps in LDS
+constexpr long long kSpinTimeoutTicks = 400000000LL;  // wall_clock64 runs at 100 MHz
+// Stencil mask of a liquid cell: bits 0-2 non-solid neighbour count, bits 3-8
+// liquid neighbour in the order -x,+x,-y,+y,-z,+z, bit 9 the cell is liquid.
+constexpr uint16_t kMaskLiquid = 1u << 9;
+}  // namespace
+
+__device__ __forceinline__ float cell_div(const StepParams& P, const float* __restrict__ u,
+                                          const float* __restrict__ v, const float* __restrict__ w,
+                                          const uint32_t* __restrict__ occ, uint32_t c) {
+    const uint32_t nx = P.nx, ny = P.ny;
+    const uint32_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const float u_hi = u[(i + 1u) + (nx + 1u) * (j + ny * k)];
+    const float u_lo = u[i + (nx + 1u) * (j + ny * k)];
+    const float v_hi = v[i + nx * ((j + 1u) + (ny + 1u) * k)];
+    const float v_lo = v[i + nx * (j + (ny + 1u) * k)];
+    const float w_hi = w[i + nx * (j + ny * (k + 1u))];
+    const float w_lo = w[i + nx * (j + ny * k)];
+    float div = ((u_hi - u_lo) + (v_hi - v_lo) + (w_hi - w_lo)) * P.inv_h;
+    if (P.stiff > 0.0f && P.rest > 0.0f) {
+        const float over = fmaxf(0.0f, (float)occ[c] - P.rest) / P.rest;
+        const float drift = fminf(P.stiff * over, P.drift);
+        div = div - drift;
+    }
+    return div;
+}
+
+__global__ void k_div_cginit(StepParams P, const float* __restrict__ u, const float* __restrict__ v,
+                             const float* __restrict__ w, const uint8_t* __restrict__ ct,
+                             const uint32_t* __restrict__ occ, float* __restrict__ div, float* __restrict__ b,
+                             float* __restrict__ p, float* __restrict__ r, float* __restrict__ d,
+                             uint16_t* __restrict__ nmask, uint32_t* __restrict__ tile_flags) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    bool liquid = false;
+    if (c < P.C) {
+        liquid = ct[c] == kLiquid;
+        if (liquid) {
+            const uint32_t nx = P.nx, nxny = P.nx * P.ny;
+            const uint32_t nb[6] = {c - 1u, c + 1u, c - nx, c + nx, c - nxny, c + nxny};
+            uint16_t m = kMaskLiquid;
+            uint32_t cnt = 0;
+            float fcnt = 0.0f, sum = 0.0f;
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                const uint8_t t = ct[nb[s]];
+                if (t != kSolid) {
+                    ++cnt;
+                    fcnt += 1.0f;
+                    if (t == kLiquid) {
+                        m |= (uint16_t)(1u << (3 + s));
+                        if (P.warm) sum += p[nb[s]];
+                    }
+                }
+            }
+            m |= (uint16_t)cnt;
+            const float dv = cell_div(P, u, v, w, occ, c);
+            const float scale = P.rho * P.h * P.h / P.dt;
+            const float rhs = -scale * dv;
+            float ap = 0.0f;
+            if (P.warm) ap = fcnt * p[c] - sum;  // sh/cg_init.wgsl:24-48
+            else p[c] = 0.0f;
+            const float rc = rhs - ap;
+            div[c] = dv;
+            b[c] = rhs;
+            r[c] = rc;
+            d[c] = rc;
+            nmask[c] = m;
+        } else {
+            div[c] = 0.0f;
+            b[c] = 0.0f;
+            p[c] = 0.0f;
+            r[c] = 0.0f;
+            d[c] = 0.0f;
+            nmask[c] = 0;
+        }
+    }
+    const uint64_t m = __ballot(liquid);
+    const uint32_t tile = c / kTile;
+    if (lane_id() == 0 && (size_t)tile * kTile < P.C) tile_flags[tile] = m ? 1u : 0u;
+}
+
+__global__ void k_compact_tiles(uint32_t ntiles, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ scan,
+                                uint32_t* __restrict__ tiles, uint32_t* __restrict__ n_active) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    if (flags[t]) tiles[scan[t]] = t;
+    if (t == ntiles - 1u) *n_active = scan[t] + flags[t];
+}
+
+hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uint8_t* ct, const uint32_t* occ,
+                             float* div, float* b, float* p, float* r, float* d, const CgWork& w, uint32_t* scan_ws,
+                             hipStream_t s) {
+    // blockDim is a multiple of 64, so each wave is exactly one tile.
+    hipLaunchKernelGGL(k_div_cginit, dim3(grid_for(P.C, 256)), dim3(256), 0, s, P, vel[0], vel[1], vel[2], ct, occ,
+                       div, b, p, r, d, w.nmask, w.tile_flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t ntiles = (P.C + kTile - 1) / kTile;
+    e = launch_exclusive_scan(w.tile_flags, w.tile_scan, ntiles, scan_ws, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, ntiles, w.tile_flags,
+                       w.tile_scan, w.tiles, w.n_tiles);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic f64 grid reductions.
+//
+// Each block sums its waves in wave order and publishes one partial.  Arrival
+// is counted in two levels (a counter per group of blocks with equal
+// blockIdx % 8, then one counter over the groups), so no single address takes
+// more than G/8 atomics.  The block arriving last sums the partials in block
+// order (a fixed tree), so the total does not depend on timing.  Ordering:
+// every wave's stores reach L2 before the workgroup barrier; thread 0's
+// release on its counter writes the XCD's L2 back; the consumer's acquire
+// invalidates its caches once.  Waiting blocks poll with relaxed loads (an
+// acquire per poll would invalidate the shared L2 on every spin).
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double* s_w) {
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) s_w[wv] = v;
+    __syncthreads();
+    double bs = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < THREADS / 64; ++i) bs += s_w[i];
+    return bs;
+}
+
+template <int THREADS>
+__device__ __forceinline__ double sum_partials(const double* partials, uint32_t n, double* s_tree) {
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += THREADS)
+        acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_tree[threadIdx.x] = acc;
+    __syncthreads();
+    for (int d = THREADS / 2; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) s_tree[threadIdx.x] += s_tree[threadIdx.x + d];
+        __syncthreads();
+    }
+    return s_tree[0];
+}
+
+// Two-level arrival (thread 0 only); returns true for the block arriving last.
+__device__ __forceinline__ bool arrive(const CgWork& w) {
+    const uint32_t G = gridDim.x;
+    const uint32_t grp = blockIdx.x & 7u;
+    const uint32_t in_grp = (G - grp + 7u) / 8u;
+    const uint32_t ngrp = G < 8u ? G : 8u;
+    uint32_t* gc = w.group_count + grp * 32u;  // one 128-byte line per group
+    const uint32_t prev = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != in_grp - 1u) return false;
+    __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t prev2 = __hip_atomic_fetch_add(w.bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev2 != ngrp - 1u) return false;
+    __hip_atomic_store(w.bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Grid barrier + sum for the persistent kernel.  Returns false on abort.
+template <int THREADS>
+__device__ bool grid_sum(double v, const CgWork& w, double& out) {
+    __shared__ double s_w[THREADS / 64];
+    __shared__ double s_tree[THREADS];
+    __shared__ int s_last, s_abort;
+    __shared__ uint32_t s_gen;
+    const double bs = block_sum<THREADS>(v, s_w);
+    if (threadIdx.x == 0) {
+        w.partials[blockIdx.x] = bs;
+        s_gen = ld_relaxed(w.bar_gen);
+        s_abort = 0;
+        s_last = arrive(w) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_last) {
+        const double tot = sum_partials<THREADS>(w.partials, gridDim.x, s_tree);
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(w.result, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(w.bar_gen, s_gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (threadIdx.x == 0) {
+        const long long t0 = wall_clock64();
+        while (ld_relaxed(w.bar_gen) == s_gen) {
+            if (ld_relaxed(w.abort_flag)) {
+                s_abort = 1;
+                break;
+            }
+            if (wall_clock64() - t0 > kSpinTimeoutTicks) {
+                __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (s_abort) return false;
+    out = __hip_atomic_load(w.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Last-block-done reduction: no block waits; the last to arrive gets the
+// total (valid in thread 0) and returns true.
+template <int THREADS>
+__device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
+    __shared__ double s_w[THREADS / 64];
+    __shared__ double s_tree[THREADS];
+    __shared__ int s_last;
+    const double bs = block_sum<THREADS>(v, s_w);
+    if (threadIdx.x == 0) {
+        w.partials[blockIdx.x] = bs;
+        s_last = arrive(w) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    tot = sum_partials<THREADS>(w.partials, gridDim.x, s_tree);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell work, kU tiles at a time: all loads of the batch are issued before
+// any result is computed, so a wave keeps several memory round trips in flight.
+struct TileWalk {
+    uint32_t begin, end, stride;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(uint32_t nt, uint32_t logical_block, uint32_t G, uint32_t waves) {
+    TileWalk t;
+    const uint32_t b0 = (uint32_t)(((uint64_t)nt * logical_block) / G);
+    t.begin = b0 + (threadIdx.x >> 6);
+    t.end = (uint32_t)(((uint64_t)nt * (logical_block + 1u)) / G);
+    t.stride = waves;
+    return t;
+}
+
+// Phase A for tiles ti, ti+stride, ... (kU of them): dn = first ? d : r + beta*d_prev,
+// q = A dn (liquid cells; 0 elsewhere); returns this lane's dn.q.
+template <typename TileId>
+__device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                                const float* __restrict__ r, const float* __restrict__ dprev,
+                                                float* __restrict__ dnext, float* __restrict__ q, float beta,
+                                                bool first, const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    uint32_t c[kU];
+    uint16_t m[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+    }
+    float rc[kU], dc[kU], rn[kU][6], dnb[kU][6];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const bool ok = c[u] < C;
+        rc[u] = (ok && !first) ? r[c[u]] : 0.0f;
+        dc[u] = ok ? dprev[c[u]] : 0.0f;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const bool lq = (m[u] >> (3 + s)) & 1u;
+            const uint32_t nbc = (uint32_t)((int32_t)c[u] + off[s]);
+            rn[u][s] = (lq && !first) ? r[nbc] : 0.0f;
+            dnb[u][s] = lq ? dprev[nbc] : 0.0f;
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        if (c[u] >= C) continue;
+        const float dn = first ? dc[u] : rc[u] + beta * dc[u];
+        float qc = 0.0f;
+        if (m[u] & kMaskLiquid) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if ((m[u] >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
+            qc = (float)(m[u] & 7u) * dn - sum;
+        }
+        if (!first) dnext[c[u]] = dn;
+        q[c[u]] = qc;
+        acc += (double)dn * (double)qc;
+    }
+    return acc;
+}
+
+template <typename TileId>
+__device__ __forceinline__ double phase_b_batch(const StepParams& P, const float* __restrict__ dn,
+                                                const float* __restrict__ q, float* __restrict__ p,
+                                                float* __restrict__ r, float alpha, const TileWalk& tw, uint32_t ti,
+                                                TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+    uint32_t c[kU];
+    float pv[kU], dv[kU], rv[kU], qv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const bool ok = c[u] < C;
+        pv[u] = ok ? p[c[u]] : 0.0f;
+        dv[u] = ok ? dn[c[u]] : 0.0f;
+        rv[u] = ok ? r[c[u]] : 0.0f;
+        qv[u] = ok ? q[c[u]] : 0.0f;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        if (c[u] >= C) continue;
+        p[c[u]] = pv[u] + alpha * dv[u];
+        const float rn = rv[u] - alpha * qv[u];
+        r[c[u]] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    return acc;
+}
+
+template <typename TileId>
+__device__ __forceinline__ double rr_batch(const StepParams& P, const float* __restrict__ r, const TileWalk& tw,
+                                           uint32_t ti, TileId&& tile_id) {
+    const uint32_t lane = lane_id();
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : P.C;
+        if (c < P.C) {
+            const double rv = (double)r[c];
+            acc += rv * rv;
+        }
+    }
+    return acc;
+}
+
+template <typename TileId>
+__device__ __forceinline__ double true_res_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                                 const float* __restrict__ b, const float* __restrict__ p,
+                                                 const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : C;
+        if (c >= C) continue;
+        const uint16_t m = nmask[c];
+        if (!(m & kMaskLiquid)) continue;
+        double sum = 0.0;
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+            if ((m >> (3 + s)) & 1u) sum += (double)p[(uint32_t)((int32_t)c + off[s])];
+        const double e = (double)b[c] - ((double)(m & 7u) * (double)p[c] - sum);
+        acc += e * e;
+    }
+    return acc;
+}
+
+// ---- persistent form ----
+__global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, const uint8_t* __restrict__ ct,
+                                                               const float* __restrict__ b, float* __restrict__ p,
+                                                               float* __restrict__ r, float* __restrict__ d0,
+                                                               float* __restrict__ d1, float* __restrict__ q,
+                                                               CgWork w) {
+    __shared__ uint32_t s_tiles[kLdsTiles];
+    (void)ct;
+    const uint32_t G = gridDim.x;
+    const uint32_t nt = *w.n_tiles;
+    const uint32_t lb = xcd_remap(blockIdx.x, G);
+    const TileWalk tw = tile_walk(nt, lb, G, kCgThreads / 64);
+    // This block's slice of the tile list, kept in LDS when it fits.
+    const uint32_t s0 = (uint32_t)(((uint64_t)nt * lb) / G);
+    const uint32_t s1 = (uint32_t)(((uint64_t)nt * (lb + 1u)) / G);
+    const bool in_lds = s1 - s0 <= (uint32_t)kLdsTiles;
+    if (in_lds)
+        for (uint32_t i = s0 + threadIdx.x; i < s1; i += kCgThreads) s_tiles[i - s0] = w.tiles[i];
+    __syncthreads();
+    const uint32_t* gt = w.tiles;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return in_lds ? s_tiles[t - s0] : gt[t]; };
+    const uint32_t step = tw.stride * kU;
+
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch(P, r, tw, ti, tile_id);
+    double rs0;
+    if (!grid_sum<kCgThreads>(acc, w, rs0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        return;
+    }
+    double rs_old = rs0;
+    const double rs_init = rs0;
+    const double tol = (double)P.tol;
+    const double tol_sq = tol * tol;
+    float beta = 0.0f;
+    float* dprev = d0;
+    float* dnext = d1;
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < P.iters; ++it) {
+        ++iters;
+        const bool first = it == 0;
+        float* dn_buf = first ? dprev : dnext;
+        acc = 0.0;
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+            acc += phase_a_batch(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
+        double dq;
+        if (!grid_sum<kCgThreads>(acc, w, dq)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+            return;
+        }
+        const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
+        acc = 0.0;
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+            acc += phase_b_batch(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
+        double rr;
+        if (!grid_sum<kCgThreads>(acc, w, rr)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+            return;
+        }
+        beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
+        rs_old = rr;
+        if (tol > 0.0 && rr <= tol_sq * rs_init) break;
+        if (!first) {
+            float* t = dprev;
+            dprev = dnext;
+            dnext = t;
+        }
+    }
+    acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch(P, w.nmask, b, p, tw, ti, tile_id);
+    double tr;
+    if (!grid_sum<kCgThreads>(acc, w, tr)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.state->rs_init = rs_init;
+        w.state->rs_old = rs_old;
+        w.state->true_res_sq = tr;
+        w.state->iterations = iters;
+        w.state->active = 0u;
+        w.state->error = 0u;
+    }
+}
+
+int cg_persistent_max_blocks() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_persistent, kCgThreads, 0) != hipSuccess)
+        return 0;
+    return cus * per_cu;
+}
+
+hipError_t launch_cg_persistent(const StepParams& P, const uint8_t* ct, const float* b, float* p, float* r,
+                                float* d0, float* d1, float* q, const CgWork& w, int grid_blocks, hipStream_t s) {
+    void* args[] = {(void*)&P, (void*)&ct, (void*)&b, (void*)&p, (void*)&r,
+                    (void*)&d0, (void*)&d1, (void*)&q, (void*)&w};
+    return hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(grid_blocks), dim3(kCgThreads), args, 0, s);
+}
+
+// ---- multi-kernel form ----
+__global__ __launch_bounds__(kCgMultiThreads) void k_cg_rs0(StepParams P, const float* __restrict__ r, CgWork w) {
+    const uint32_t G = gridDim.x;
+    const TileWalk tw = tile_walk(*w.n_tiles, xcd_remap(blockIdx.x, G), G, kCgMultiThreads / 64);
+    const uint32_t* gt = w.tiles;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU) acc += rr_batch(P, r, tw, ti, tile_id);
+    double tot;
+    if (last_block_sum<kCgMultiThreads>(acc, w, tot) && threadIdx.x == 0) {
+        CgState* st = w.state;
+        st->rs_old = tot;
+        st->rs_init = tot;
+        st->alpha = 0.0f;
+        st->beta = 0.0f;
+        st->iterations = 0u;
+        st->active = 1u;
+        st->error = 0u;
+    }
+}
+
+__global__ __launch_bounds__(kCgMultiThreads) void k_cg_phase_a(StepParams P, const float* __restrict__ r,
+                                                                 const float* __restrict__ dprev,
+                                                                 float* __restrict__ dnext, float* __restrict__ q,
+                                                                 int first, CgWork w) {
+    CgState* st = w.state;
+    if (st->active == 0u) return;
+    const float beta = st->beta;
+    const uint32_t G = gridDim.x;
+    const TileWalk tw = tile_walk(*w.n_tiles, xcd_remap(blockIdx.x, G), G, kCgMultiThreads / 64);
+    const uint32_t* gt = w.tiles;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
+        acc += phase_a_batch(P, w.nmask, r, dprev, dnext, q, beta, first != 0, tw, ti, tile_id);
+    double dq;
+    if (last_block_sum<kCgMultiThreads>(acc, w, dq) && threadIdx.x == 0) {
+        st->alpha = fabs(dq) > 1e-30 ? (float)(st->rs_old / dq) : 0.0f;
+        st->iterations += 1u;
+    }
+}
+
+__global__ __launch_bounds__(kCgMultiThreads) void k_cg_phase_b(StepParams P, const float* __restrict__ dn,
+                                                                 const float* __restrict__ q, float* __restrict__ p,
+                                                                 float* __restrict__ r, CgWork w) {
+    CgState* st = w.state;
+    if (st->active == 0u) return;
+    const float alpha = st->alpha;
+    const uint32_t G = gridDim.x;
+    const TileWalk tw = tile_walk(*w.n_tiles, xcd_remap(blockIdx.x, G), G, kCgMultiThreads / 64);
+    const uint32_t* gt = w.tiles;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
+        acc += phase_b_batch(P, dn, q, p, r, alpha, tw, ti, tile_id);
+    double rr;
+    if (last_block_sum<kCgMultiThreads>(acc, w, rr) && threadIdx.x == 0) {
+        const double rs_old = st->rs_old;
+        st->beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
+        st->rs_old = rr;
+        const double tol = (double)P.tol;
+        if (tol > 0.0 && rr <= tol * tol * st->rs_init) st->active = 0u;
+    }
+}
+
+__global__ __launch_bounds__(kCgMultiThreads) void k_cg_true_res(StepParams P, const float* __restrict__ b,
+                                                                  const float* __restrict__ p, CgWork w) {
+    const uint32_t G = gridDim.x;
+    const TileWalk tw = tile_walk(*w.n_tiles, xcd_remap(blockIdx.x, G), G, kCgMultiThreads / 64);
+    const uint32_t* gt = w.tiles;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
+        acc += true_res_batch(P, w.nmask, b, p, tw, ti, tile_id);
+    double tr;
+    if (last_block_sum<kCgMultiThreads>(acc, w, tr) && threadIdx.x == 0) {
+        w.state->true_res_sq = tr;
+        w.state->active = 0u;
+    }
+}
+
+hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* b, float* p, float* r, float* d0,
+                           float* d1, float* q, const CgWork& w, int grid_blocks, hipStream_t s) {
+    (void)ct;
+    const dim3 g(grid_blocks), t(kCgMultiThreads);
+    hipLaunchKernelGGL(k_cg_rs0, g, t, 0, s, P, r, w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    float* D[2] = {d0, d1};
+    for (uint32_t it = 0; it < P.iters; ++it) {
+        const int first = it == 0;
+        const float* dprev = first ? d0 : D[(it + 1u) & 1u];
+        float* dnext = first ? d1 : D[it & 1u];
+        const float* dn = first ? d0 : D[it & 1u];
+        hipLaunchKernelGGL(k_cg_phase_a, g, t, 0, s, P, r, dprev, dnext, q, first, w);
+        hipLaunchKernelGGL(k_cg_phase_b, g, t, 0, s, P, dn, q, p, r, w);
+    }
+    hipLaunchKernelGGL(k_cg_true_res, g, t, 0, s, P, b, p, w);
+    return hipGetLastError();
+}
+
+}  // namespace fs
