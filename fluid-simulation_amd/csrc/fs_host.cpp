@@ -641,7 +641,8 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(t->alloc(t->cg.nmask, C));
         FS_HIP(t->alloc(t->cg.bar_gen, 1));
         FS_HIP(t->alloc(t->cg.abort_flag, 1));
-        FS_HIP(t->alloc(t->cg.partials, (size_t)std::max(t->cg_blocks, t->cg_multi_blocks)));
+        // Two halves: the persistent form alternates them between reductions.
+        FS_HIP(t->alloc(t->cg.partials, 2 * (size_t)std::max(t->cg_blocks, t->cg_multi_blocks)));
         FS_HIP(t->alloc(t->cg.result, 4));
         FS_HIP(t->alloc(t->cg.tile_flags, ntiles));
         FS_HIP(t->alloc(t->cg.tile_scan, ntiles));

@@ -179,22 +179,32 @@ __device__ __forceinline__ double sum_partials(const double* partials, uint32_t 
 }
 
 // Two-level arrival (thread 0 only); returns true for the block arriving last.
+// Arrivals are release RMWs; only a block that turns out to be a group's or
+// the grid's last arrival issues the acquire fence (fence-after-RMW
+// synchronizes with every release earlier in the counter's RMW chain).
 __device__ __forceinline__ bool arrive(const CgWork& w) {
     const uint32_t G = gridDim.x;
     const uint32_t grp = blockIdx.x & 7u;
     const uint32_t in_grp = (G - grp + 7u) / 8u;
     const uint32_t ngrp = G < 8u ? G : 8u;
     uint32_t* gc = w.group_count + grp * 32u;  // one 128-byte line per group
-    const uint32_t prev = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t prev = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (prev != in_grp - 1u) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t prev2 = __hip_atomic_fetch_add(w.bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t prev2 = __hip_atomic_fetch_add(w.bar_count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (prev2 != ngrp - 1u) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __hip_atomic_store(w.bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
-// Grid barrier + sum for the persistent kernel.  Returns false on abort.
+// Grid barrier + sum for the persistent kernel.  The last arrival only bumps
+// the generation word; afterwards every block sums the G published partials
+// itself in the same fixed order, so all blocks hold the identical total
+// without another round trip.  Partials alternate between two halves of the
+// array so a fast block's next publication cannot overwrite a value a slow
+// block is still summing.  Returns false on abort.
 template <int THREADS>
 __device__ bool grid_sum(double v, const CgWork& w, double& out) {
     __shared__ double s_w[THREADS / 64];
@@ -203,18 +213,16 @@ __device__ bool grid_sum(double v, const CgWork& w, double& out) {
     __shared__ uint32_t s_gen;
     const double bs = block_sum<THREADS>(v, s_w);
     if (threadIdx.x == 0) {
-        w.partials[blockIdx.x] = bs;
-        s_gen = ld_relaxed(w.bar_gen);
+        const uint32_t g0 = ld_relaxed(w.bar_gen);
+        w.partials[(size_t)(g0 & 1u) * gridDim.x + blockIdx.x] = bs;
+        s_gen = g0;
         s_abort = 0;
         s_last = arrive(w) ? 1 : 0;
     }
     __syncthreads();
     if (s_last) {
-        const double tot = sum_partials<THREADS>(w.partials, gridDim.x, s_tree);
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(w.result, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0)
             __hip_atomic_store(w.bar_gen, s_gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
     } else if (threadIdx.x == 0) {
         const long long t0 = wall_clock64();
         while (ld_relaxed(w.bar_gen) == s_gen) {
@@ -233,7 +241,7 @@ __device__ bool grid_sum(double v, const CgWork& w, double& out) {
     }
     __syncthreads();
     if (s_abort) return false;
-    out = __hip_atomic_load(w.result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    out = sum_partials<THREADS>(w.partials + (size_t)(s_gen & 1u) * gridDim.x, gridDim.x, s_tree);
     return true;
 }
 
