@@ -168,7 +168,7 @@ struct fs_tank {
     uint32_t* counters = nullptr;  // [0] liquid cells, [1] misfits, [2] CG iterations (reference engine)
     CgWork cg{};
     int cg_blocks = 0;        // persistent form
-    int cg_mode = 1;          // 0 persistent kernel, 1 two kernels per iteration
+    int cg_mode = 0;          // 0 persistent kernel, 1 two kernels per iteration
     int cg_multi_blocks = 0;  // multi-kernel form
     int cg_use_graph = 1;     // replay the multi-kernel solve from a captured hipGraph
     hipGraphExec_t cg_exec = nullptr;

@@ -39,13 +39,19 @@ __global__ void k_classify(StepParams P, const uint32_t* __restrict__ occ, uint8
             ct[c] = liquid ? kLiquid : kAir;
         }
     }
+    // One atomic per block (the count is an integer sum, so grouping is free).
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
     const uint64_t m = __ballot(liquid);
-    if (lane_id() == 0 && m) atomicAdd(stats, (uint32_t)__popcll(m));
+    if (lane_id() == 0 && m) atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_cnt) atomicAdd(stats, s_cnt);
 }
 
 hipError_t launch_classify(const StepParams& P, const uint32_t* occ, uint8_t* ct, uint32_t* stats,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_classify, dim3(grid_for(P.C, 256)), dim3(256), 0, s, P, occ, ct, stats);
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(P.C, 1024)), dim3(1024), 0, s, P, occ, ct, stats);
     return hipGetLastError();
 }
 
@@ -341,7 +347,12 @@ __global__ __launch_bounds__(kGThreads) void k_p2g_gather(StepParams P, Particle
     const int32_t owny_hi = (y0 + kGBY >= ny) ? kGBY : kGBY - 1;
     const int32_t ownz_hi = (z0 + kGBZ >= nz) ? kGBZ : kGBZ - 1;
     int row = 0;
-    for (uint32_t t = threadIdx.x; t < total; t += kGThreads) {
+    for (uint32_t base = 0; base < total; base += kGThreads) {
+        // Each wave takes every 4th item of the chunk, so its lanes fall in
+        // different cells (sorted particles of one cell share faces, and
+        // same-address LDS atomics within a wave serialize).
+        const uint32_t t = base + (threadIdx.x & 63u) * (kGThreads / 64) + (threadIdx.x >> 6);
+        if (t >= total) continue;
         // Find the row holding flattened item t (rows are few; linear advance).
         while (s_row_pref[row + 1] <= t) ++row;
         const uint32_t p = s_row_start[row] + (t - s_row_pref[row]);
