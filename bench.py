@@ -161,7 +161,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--bf16", action="store_true", help="bf16 particle-velocity storage")
@@ -169,6 +169,9 @@ def main():
                     help="instead of timing, report each GPU precision mode's error against the FP64 "
                          "restatement after STEPS steps (and one-step errors from its states)")
     ap.add_argument("--teacher-states", type=int, default=3)
+    ap.add_argument("--workload", default="high_density_128",
+                    choices=["high_density_128", "large_splash_256", "dam_break_32", "control_64"],
+                    help="the metric is quoted on high_density_128; the others are for per-kernel profiles")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -186,7 +189,7 @@ def main():
         print(json.dumps(precision_report(args.precision_report, args.teacher_states)), flush=True)
         return
 
-    sc = scenes.high_density_128()
+    sc = getattr(scenes, args.workload)()
     N = len(sc["pos"])
     vstore = fsm.VELOCITY_BF16 if args.bf16 else fsm.VELOCITY_F32
     fused = gpu_params(sc, fsm.ENGINE_FUSED, velocity=vstore)
@@ -226,8 +229,8 @@ def main():
                      % (" (bf16 particle velocity)" if args.bf16 else ""),
         "data": "synthetic",
         "config": {
-            "workload": "high_density_128", "grid": [128, 128, 128], "particles": N, "dt": 1 / 240,
-            "flip": 0.95, "cg_tolerance": 1e-5, "cg_cap": 200, "sort_period": 1, "engine": "fused",
+            "workload": sc["name"], "grid": [sc["n"]] * 3, "particles": N, "dt": sc["physics"]["dt"],
+            "flip": sc["physics"]["flip_blend"], "cg_tolerance": 1e-5, "cg_cap": 200, "sort_period": 1, "engine": "fused",
         },
         "cg": {"iterations_per_solve": iters, "active_cells": act, "ms_per_solve": cg_ms,
                "ms_per_iteration": [m / max(k, 1) for m, k in zip(cg_ms, iters)]},
