@@ -143,8 +143,15 @@ def precision_report(steps, teacher_states):
             t.load_state(sp.astype(np.float32), sv.astype(np.float32), spr.astype(np.float32))
             t.step(1)
             gp, gv = t.particles()
-            one["pos"].append(rel_l2(gp, r.pos))
-            one["vel"].append(rel_l2(gv, r.vel_particles))
+            # Both sides re-sorted from the same input order; undo each side's
+            # permutation so particles are compared by identity.
+            gperm, operm = t.permutation().astype(np.int64), r.perm.astype(np.int64)
+            ug, uo = np.empty_like(gp), np.empty_like(r.pos)
+            ug[gperm], uo[operm] = gp, r.pos
+            uvg, uvo = np.empty_like(gv), np.empty_like(r.vel_particles)
+            uvg[gperm], uvo[operm] = gv, r.vel_particles
+            one["pos"].append(rel_l2(ug, uo))
+            one["vel"].append(rel_l2(uvg, uvo))
             one["pressure"].append(rel_l2(t.pressure(), r.pressure))
         for k, v in one.items():
             if v:
@@ -212,6 +219,8 @@ def main():
         act.append(Ca)
     achieved = float(np.mean(ach)) if ach else None
     traffic = load_traffic()
+    if traffic and traffic.get("workload") != sc["name"]:
+        traffic = None
 
     line = {
         "metric": "FLIP particle-steps/s @128^3: mixed vs FP32 CG at 1e-5 residual; CG %HBM peak",
