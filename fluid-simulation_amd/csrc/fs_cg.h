@@ -1,0 +1,187 @@
+// fs_cg.h — building blocks of the mixed-precision CG shared by the
+// single-tank and batched solvers: deterministic f64 block/partial sums and
+// the per-tile work of one iteration (kU tiles of 64 cells per wave step).
+#pragma once
+#include "fs_kernels.h"
+#include "fs_face.h"
+
+namespace fs {
+
+constexpr int kTile = 64;  // cells per CG tile (one wave)
+constexpr int kU = 4;      // default tiles a wave loads before computing (memory-level parallelism)
+constexpr long long kSpinTimeoutTicks = 400000000LL;  // wall_clock64 runs at 100 MHz
+// Stencil mask of a liquid cell: bits 0-2 non-solid neighbour count, bits 3-8
+// liquid neighbour in the order -x,+x,-y,+y,-z,+z, bit 9 the cell is liquid.
+constexpr uint16_t kMaskLiquid = 1u << 9;
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double* s_w) {
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) s_w[wv] = v;
+    __syncthreads();
+    double bs = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < THREADS / 64; ++i) bs += s_w[i];
+    return bs;
+}
+
+template <int THREADS>
+__device__ __forceinline__ double sum_partials(const double* partials, uint32_t n, double* s_tree) {
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += THREADS)
+        acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_tree[threadIdx.x] = acc;
+    __syncthreads();
+    for (int d = THREADS / 2; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) s_tree[threadIdx.x] += s_tree[threadIdx.x + d];
+        __syncthreads();
+    }
+    return s_tree[0];
+}
+
+// Per-cell work, U tiles at a time: all loads of the batch are issued before
+// any result is computed, so a wave keeps several memory round trips in flight.
+struct TileWalk {
+    uint32_t begin, end, stride;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(uint32_t nt, uint32_t logical_block, uint32_t G, uint32_t waves) {
+    TileWalk t;
+    const uint32_t b0 = (uint32_t)(((uint64_t)nt * logical_block) / G);
+    t.begin = b0 + (threadIdx.x >> 6);
+    t.end = (uint32_t)(((uint64_t)nt * (logical_block + 1u)) / G);
+    t.stride = waves;
+    return t;
+}
+
+// Phase A for tiles ti, ti+stride, ... (U of them): dn = first ? d : r + beta*d_prev,
+// q = A dn (liquid cells; 0 elsewhere); returns this lane's dn.q.
+template <int U, typename TileId>
+__device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                                const float* __restrict__ r, const float* __restrict__ dprev,
+                                                float* __restrict__ dnext, float* __restrict__ q, float beta,
+                                                bool first, const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    uint32_t c[U];
+    uint16_t m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+    }
+    float rc[U], dc[U], rn[U][6], dnb[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const bool ok = c[u] < C;
+        rc[u] = (ok && !first) ? r[c[u]] : 0.0f;
+        dc[u] = ok ? dprev[c[u]] : 0.0f;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const bool lq = (m[u] >> (3 + s)) & 1u;
+            const uint32_t nbc = (uint32_t)((int32_t)c[u] + off[s]);
+            rn[u][s] = (lq && !first) ? r[nbc] : 0.0f;
+            dnb[u][s] = lq ? dprev[nbc] : 0.0f;
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (c[u] >= C) continue;
+        const float dn = first ? dc[u] : rc[u] + beta * dc[u];
+        float qc = 0.0f;
+        if (m[u] & kMaskLiquid) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if ((m[u] >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
+            qc = (float)(m[u] & 7u) * dn - sum;
+        }
+        if (!first) dnext[c[u]] = dn;
+        q[c[u]] = qc;
+        acc += (double)dn * (double)qc;
+    }
+    return acc;
+}
+
+template <int U, typename TileId>
+__device__ __forceinline__ double phase_b_batch(const StepParams& P, const float* __restrict__ dn,
+                                                const float* __restrict__ q, float* __restrict__ p,
+                                                float* __restrict__ r, float alpha, const TileWalk& tw, uint32_t ti,
+                                                TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+    uint32_t c[U];
+    float pv[U], dv[U], rv[U], qv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const bool ok = c[u] < C;
+        pv[u] = ok ? p[c[u]] : 0.0f;
+        dv[u] = ok ? dn[c[u]] : 0.0f;
+        rv[u] = ok ? r[c[u]] : 0.0f;
+        qv[u] = ok ? q[c[u]] : 0.0f;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (c[u] >= C) continue;
+        p[c[u]] = pv[u] + alpha * dv[u];
+        const float rn = rv[u] - alpha * qv[u];
+        r[c[u]] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    return acc;
+}
+
+template <int U, typename TileId>
+__device__ __forceinline__ double rr_batch(const StepParams& P, const float* __restrict__ r, const TileWalk& tw,
+                                           uint32_t ti, TileId&& tile_id) {
+    const uint32_t lane = lane_id();
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : P.C;
+        if (c < P.C) {
+            const double rv = (double)r[c];
+            acc += rv * rv;
+        }
+    }
+    return acc;
+}
+
+template <int U, typename TileId>
+__device__ __forceinline__ double true_res_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                                 const float* __restrict__ b, const float* __restrict__ p,
+                                                 const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : C;
+        if (c >= C) continue;
+        const uint16_t m = nmask[c];
+        if (!(m & kMaskLiquid)) continue;
+        double sum = 0.0;
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+            if ((m >> (3 + s)) & 1u) sum += (double)p[(uint32_t)((int32_t)c + off[s])];
+        const double e = (double)b[c] - ((double)(m & 7u) * (double)p[c] - sum);
+        acc += e * e;
+    }
+    return acc;
+}
+
+}  // namespace fs

@@ -130,4 +130,14 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
     return (b % X) * (G / X) + b / X;
 }
 
+// Splits an index of the concatenated u|v|w face space into (axis, face).
+__device__ __forceinline__ bool split_face(const StepParams& P, uint32_t g, int& a, uint32_t& f) {
+    if (g < P.F[0]) { a = 0; f = g; return true; }
+    g -= P.F[0];
+    if (g < P.F[1]) { a = 1; f = g; return true; }
+    g -= P.F[1];
+    if (g < P.F[2]) { a = 2; f = g; return true; }
+    return false;
+}
+
 }  // namespace fs

@@ -1,0 +1,177 @@
+// fs_gather.h — the sorted-segment P2G gather of one brick of cells, shared by
+// the single-tank and the batched kernels.
+//
+// One workgroup owns a brick of cells (Brick<V>) and every MAC face
+// whose cell coordinate (own-axis coordinate clamped to n-1) lies in the
+// brick.  A face's taps come from particles whose cell is in the brick grown
+// by one cell on every side; after the sort those particles are, per (y,z)
+// row of that halo region, one contiguous segment of the sorted arrays.  The
+// workgroup walks the segments, evaluates each particle's 24 taps exactly as
+// sh/scatter.wgsl:78-118 does, and adds the ones it owns into LDS with
+// integer atomics (order-free, so the sums equal the reference's i32 atomics
+// bit for bit).  The epilogue normalizes, saves, applies gravity and enforces
+// walls per owned face (fs_face.h), and stores vel/saved once.
+//
+// Particles whose P2G cell floor((pos-origin)*inv_h) differs from their sort
+// cell (clamped floor((pos-origin)/h); the two differ by one ulp-level
+// rounding when h is not a power of two) are listed by the key pass.  A
+// listed particle whose sort cell lies outside the halo may still reach the
+// brick, so every workgroup also walks the list and adds those particles'
+// owned taps; particles inside the halo were already counted by the segment
+// walk.  Each (particle, face) term is therefore added exactly once.
+#pragma once
+#include "fs_particle.h"
+
+namespace fs {
+
+constexpr int kGThreads = 256;
+
+// Brick shapes of the gather menu: 0 = 16x8x8, 1 = 8x8x8, 2 = 32x4x4 cells.
+template <int V> struct Brick;
+template <> struct Brick<0> { static constexpr int X = 16, Y = 8, Z = 8; };
+template <> struct Brick<1> { static constexpr int X = 8, Y = 8, Z = 8; };
+template <> struct Brick<2> { static constexpr int X = 32, Y = 4, Z = 4; };
+
+struct GatherOut {
+    float* vel[3];
+    float* saved[3];
+};
+
+// Brick at cell (x0, y0, z0).  sp: the sorted side; src: the side the keys
+// were computed on (listed indices refer to it); [pbeg, pend): the particle
+// range of this tank in both sides.
+template <int BF16, int V>
+__device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const ParticleSoA& sp, const ParticleSoA& src,
+                                                 const uint32_t* __restrict__ occ, const uint32_t* __restrict__ off,
+                                                 const uint8_t* __restrict__ ct, const GatherOut& out, int32_t x0,
+                                                 int32_t y0, int32_t z0, const uint32_t* __restrict__ mis_list,
+                                                 uint32_t mis_n, uint32_t pbeg, uint32_t pend) {
+    constexpr int kGBX = Brick<V>::X, kGBY = Brick<V>::Y, kGBZ = Brick<V>::Z;
+    constexpr int kGRowsMax = (kGBY + 2) * (kGBZ + 2);
+    constexpr int kGU = (kGBX + 1) * kGBY * kGBZ;
+    constexpr int kGV = kGBX * (kGBY + 1) * kGBZ;
+    constexpr int kGW = kGBX * kGBY * (kGBZ + 1);
+    __shared__ int32_t s_num[kGU + kGV + kGW];
+    __shared__ int32_t s_den[kGU + kGV + kGW];
+    __shared__ uint32_t s_row_start[kGRowsMax];
+    __shared__ uint32_t s_row_pref[kGRowsMax + 1];
+    const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
+
+    for (int t = threadIdx.x; t < kGU + kGV + kGW; t += kGThreads) {
+        s_num[t] = 0;
+        s_den[t] = 0;
+    }
+    // Halo cell range (inclusive), clamped to the grid.
+    const int32_t xs = max(x0 - 1, 0), xe = min(x0 + kGBX, nx - 1);
+    const int32_t ys = max(y0 - 1, 0), ye = min(y0 + kGBY, ny - 1);
+    const int32_t zs = max(z0 - 1, 0), ze = min(z0 + kGBZ, nz - 1);
+    const int32_t rows_y = ye - ys + 1;
+    const int32_t rows = rows_y * (ze - zs + 1);
+    if (threadIdx.x < 64) {
+        // Row segments and their prefix sums (one wave, rows <= kGRowsMax).
+        uint32_t carry = 0;
+        for (int r0 = 0; r0 < rows; r0 += 64) {
+            const int r = r0 + (int)threadIdx.x;
+            uint32_t len = 0, start = 0;
+            if (r < rows) {
+                const int32_t cy = ys + r % rows_y, cz = zs + r / rows_y;
+                const uint32_t cf = (uint32_t)xs + P.nx * ((uint32_t)cy + P.ny * (uint32_t)cz);
+                const uint32_t cl = (uint32_t)xe + P.nx * ((uint32_t)cy + P.ny * (uint32_t)cz);
+                start = off[cf];
+                len = off[cl] + occ[cl] - start;
+                s_row_start[r] = start;
+            }
+            uint32_t inc = len;
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t t = __shfl_up(inc, o, 64);
+                if ((int)threadIdx.x >= o) inc += t;
+            }
+            if (r < rows) s_row_pref[r + 1] = carry + inc;
+            carry += __shfl(inc, 63, 64);
+        }
+        if (threadIdx.x == 0) s_row_pref[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = s_row_pref[rows];
+    const int32_t ownx_hi = (x0 + kGBX >= nx) ? kGBX : kGBX - 1;  // own-axis local max index per axis
+    const int32_t owny_hi = (y0 + kGBY >= ny) ? kGBY : kGBY - 1;
+    const int32_t ownz_hi = (z0 + kGBZ >= nz) ? kGBZ : kGBZ - 1;
+
+    auto add_particle = [&](const float px, const float py, const float pz, const float pv[3]) {
+        const float pos[3] = {(px - P.origin[0]) * P.inv_h, (py - P.origin[1]) * P.inv_h,
+                              (pz - P.origin[2]) * P.inv_h};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
+            const int32_t hx = a == 0 ? ownx_hi : kGBX - 1;
+            const int32_t hy = a == 1 ? owny_hi : kGBY - 1;
+            const int32_t hz = a == 2 ? ownz_hi : kGBZ - 1;
+            const int32_t lx_dim = a == 0 ? kGBX + 1 : kGBX;
+            const int32_t ly_dim = a == 1 ? kGBY + 1 : kGBY;
+            const int32_t base = a == 0 ? 0 : (a == 1 ? kGU : kGU + kGV);
+            p2g_taps(g, dim, pv[a], P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
+                const int32_t lx = ii - x0, ly = jj - y0, lz = kk - z0;
+                if (lx < 0 || lx > hx || ly < 0 || ly > hy || lz < 0 || lz > hz) return;
+                const int32_t li = base + lx + lx_dim * (ly + ly_dim * lz);
+                atomicAdd(&s_num[li], tn);
+                atomicAdd(&s_den[li], td);
+            });
+        }
+    };
+
+    int row = 0;
+    for (uint32_t base = 0; base < total; base += kGThreads) {
+        // Each wave takes every 4th item of the chunk, so its lanes fall in
+        // different cells (sorted particles of one cell share faces, and
+        // same-address LDS atomics within a wave serialize).
+        const uint32_t t = base + (threadIdx.x & 63u) * (kGThreads / 64) + (threadIdx.x >> 6);
+        if (t >= total) continue;
+        // Find the row holding flattened item t (rows are few; linear advance).
+        while (s_row_pref[row + 1] <= t) ++row;
+        const uint32_t p = s_row_start[row] + (t - s_row_pref[row]);
+        const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
+                             load_vel<BF16>(sp.vel[2], p)};
+        add_particle(sp.pos[0][p], sp.pos[1][p], sp.pos[2][p], pv);
+    }
+    for (uint32_t e = threadIdx.x; e < mis_n; e += kGThreads) {
+        const uint32_t p = mis_list[e];
+        if (p < pbeg || p >= pend) continue;
+        const float x = src.pos[0][p], y = src.pos[1][p], z = src.pos[2][p];
+        const uint32_t key = cell_key(P, x, y, z);
+        const int32_t ci = (int32_t)(key % P.nx), cj = (int32_t)((key / P.nx) % P.ny), ck = (int32_t)(key / (P.nx * P.ny));
+        if (ci >= xs && ci <= xe && cj >= ys && cj <= ye && ck >= zs && ck <= ze) continue;
+        const float pv[3] = {load_vel<BF16>(src.vel[0], p), load_vel<BF16>(src.vel[1], p),
+                             load_vel<BF16>(src.vel[2], p)};
+        add_particle(x, y, z, pv);
+    }
+    __syncthreads();
+    // Epilogue over owned faces of the three axes.
+    for (int a = 0; a < 3; ++a) {
+        const int32_t lxd = a == 0 ? kGBX + 1 : kGBX;
+        const int32_t lyd = a == 1 ? kGBY + 1 : kGBY;
+        const int32_t lzd = a == 2 ? kGBZ + 1 : kGBZ;
+        const int32_t hx = a == 0 ? ownx_hi : kGBX - 1;
+        const int32_t hy = a == 1 ? owny_hi : kGBY - 1;
+        const int32_t hz = a == 2 ? ownz_hi : kGBZ - 1;
+        const int32_t base = a == 0 ? 0 : (a == 1 ? kGU : kGU + kGV);
+        float* vel = out.vel[a];
+        float* sav = out.saved[a];
+        uint32_t dim[3];
+        face_dims(P, a, dim);
+        for (int t = threadIdx.x; t < lxd * lyd * lzd; t += kGThreads) {
+            const int32_t lx = t % lxd, ly = (t / lxd) % lyd, lz = t / (lxd * lyd);
+            if (lx > hx || ly > hy || lz > hz) continue;
+            const int32_t i = x0 + lx, j = y0 + ly, k = z0 + lz;
+            if (i >= (int32_t)dim[0] || j >= (int32_t)dim[1] || k >= (int32_t)dim[2]) continue;
+            float vv, ss;
+            face_epilogue(P, a, ct, (uint32_t)i, (uint32_t)j, (uint32_t)k, s_num[base + t], s_den[base + t], vv, ss);
+            const uint32_t f = (uint32_t)i + dim[0] * ((uint32_t)j + dim[1] * (uint32_t)k);
+            vel[f] = vv;
+            sav[f] = ss;
+        }
+    }
+}
+
+}  // namespace fs

@@ -17,27 +17,22 @@
 
 #include "fluidstep.h"
 #include "fs_kernels.h"
+#include "fs_host_common.h"
 
 using namespace fs;
 
 // ---------------------------------------------------------------------------
 // Error reporting
-namespace {
+namespace fs {
 thread_local std::string g_last_error;
 
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+}  // namespace fs
 
-#define FS_HIP(call)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(e_ == hipErrorOutOfMemory ? FS_ERR_OUT_OF_MEMORY : FS_ERR_DEVICE,    \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
-
+namespace {
 // ---------------------------------------------------------------------------
 // Registry rows of the step (src/settings/mod.rs:407-732, 1349-1391): id,
 // type, default, inclusive bounds, apply class.
@@ -166,6 +161,7 @@ struct fs_tank {
     float* scal = nullptr;      // reference CG scalars [8]
     float* partials = nullptr;  // reference CG partials
     uint32_t* counters = nullptr;  // [0] liquid cells, [1] misfits, [2] CG iterations (reference engine)
+    uint32_t* mis_list = nullptr;  // indices (pre-sort side) of the first kMisfitCap misfit particles
     CgWork cg{};
     int cg_blocks = 0;        // persistent form
     int cg_mode = 0;          // 0 persistent kernel, 1 two kernels per iteration
@@ -208,7 +204,7 @@ struct fs_tank {
     }
 };
 
-namespace {
+namespace fs {
 
 // Params uniform fill (src/gpu/fluid.rs:289-318) + live clamps of the
 // setters (src/gpu/fluid.rs:1295-1371).
@@ -251,6 +247,43 @@ bool all_finite(const float* a, size_t n) {
         if (!std::isfinite(a[i])) return false;
     return true;
 }
+
+// Creation-time checks shared by fs_create and fs_batch_create.
+int check_params(const fs_params& q, uint32_t n, const float* pos_xyz, const float* vel_xyz) {
+    if (q.nx < 3 || q.ny < 3 || q.nz < 3) return fail(FS_ERR_INVALID_ARGUMENT, "grid must be at least 3 cells per axis");
+    if ((uint64_t)(q.nx + 1) * (q.ny + 1) * (q.nz + 1) >= (1ull << 31))
+        return fail(FS_ERR_INVALID_ARGUMENT, "grid too large for 32-bit face indices");
+    if (!(q.h > 0.0f) || !std::isfinite(q.h) || !(q.dt > 0.0f) || !std::isfinite(q.dt) || !(q.rho > 0.0f))
+        return fail(FS_ERR_INVALID_ARGUMENT, "h, dt and rho must be positive and finite");
+    const float scalars[] = {q.origin[0], q.origin[1], q.origin[2], q.gravity[0], q.gravity[1], q.gravity[2],
+                             q.flip_blend, q.wall_friction, q.rest_density, q.volume_stiffness, q.drift_clamp,
+                             q.cfl, q.pressure_residual_tolerance, q.rho};
+    if (!all_finite(scalars, sizeof scalars / sizeof scalars[0]))
+        return fail(FS_ERR_NOT_FINITE, "non-finite parameter");
+    if (q.engine > FS_ENGINE_FUSED || q.precision > FS_PRECISION_MIXED || q.velocity_storage > FS_VELOCITY_BF16)
+        return fail(FS_ERR_INVALID_ARGUMENT, "unknown engine/precision/velocity_storage");
+    if (q.engine == FS_ENGINE_REFERENCE && q.precision != FS_PRECISION_F32)
+        return fail(FS_ERR_UNSUPPORTED, "the reference engine runs FS_PRECISION_F32 only");
+    if (q.engine == FS_ENGINE_FUSED && q.precision != FS_PRECISION_MIXED)
+        return fail(FS_ERR_UNSUPPORTED, "the fused engine runs FS_PRECISION_MIXED only");
+    if (n > 0 && (!all_finite(pos_xyz, 3 * (size_t)n) || !all_finite(vel_xyz, vel_xyz ? 3 * (size_t)n : 0)))
+        return fail(FS_ERR_NOT_FINITE, "non-finite particle position or velocity");
+    if ((uint64_t)n > (1ull << 31)) return fail(FS_ERR_INVALID_ARGUMENT, "too many particles");
+    return FS_OK;
+}
+
+// Co-resident grid for a persistent kernel: the occupancy figure is not a
+// residency guarantee (on MI355X a grid of the full figure did not become
+// resident at once and its grid barrier timed out, while half of it did), so
+// half of it, in whole XCD groups (multiple of 8 blocks).
+int half_resident_blocks(int max_blocks) {
+    int b = (max_blocks / 2 / 8) * 8;
+    return b > 0 ? b : 1;
+}
+
+}  // namespace fs
+
+namespace {
 
 int upload_particles(fs_tank* t, const float* pos, const float* vel) {
     const uint32_t n = t->N;
@@ -309,13 +342,13 @@ int substep_reference(fs_tank* t, bool sort_now) {
     if (sort_now) {
         // Keys for the sort: same expression as mark; the histogram it also
         // produces goes to scratch (cell_offset is rewritten by the scan).
-        FS_HIP(launch_key_hist(P, t->side[t->cur].pos, t->keys, t->cell_offset, t->counters + 1, s));
+        FS_HIP(launch_key_hist(P, t->side[t->cur].pos, t->keys, t->cell_offset, t->counters + 1, nullptr, 0u, s));
         int rc = sort_particles(t);
         if (rc != FS_OK) return rc;
     }
     ParticleSoA live = t->side[t->cur];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
-    FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, s));
+    FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, 0u, s));
     for (int a = 0; a < 3; ++a) FS_HIP(launch_normalize(P, a, t->num[a], t->den[a], t->vel[a], s));
     for (int a = 0; a < 3; ++a) FS_HIP(launch_copy_f32(t->vel[a], t->saved[a], P.F[a], s));
     for (int a = 0; a < 3; ++a) FS_HIP(launch_forces(P, a, t->ct, t->vel[a], s));
@@ -355,7 +388,7 @@ int substep_fused(fs_tank* t, bool sort_now) {
     if (t->timing) FS_HIP(hipEventRecord(t->ev[0], s));
     FS_HIP(launch_fill_u32(t->occ, 0u, P.C, s));
     FS_HIP(launch_fill_u32(t->counters, 0u, 4, s));
-    FS_HIP(launch_key_hist(P, t->side[t->cur].pos, t->keys, t->occ, t->counters + 1, s));
+    FS_HIP(launch_key_hist(P, t->side[t->cur].pos, t->keys, t->occ, t->counters + 1, t->mis_list, kMisfitCap, s));
     if (sort_now) {
         int rc = sort_particles(t);
         if (rc != FS_OK) return rc;
@@ -363,23 +396,24 @@ int substep_fused(fs_tank* t, bool sort_now) {
     ParticleSoA live = t->side[t->cur];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
     if (sort_now) {
-        FS_HIP(launch_p2g_gather(P, live, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
-                                 t->counters + 1, s));
-        // Fallback when some particle's P2G cell differs from its sort cell.
+        FS_HIP(launch_p2g_gather(P, live, t->side[t->cur ^ 1], t->bf16, t->occ, t->cell_offset, t->ct, t->vel,
+                                 t->saved, t->counters + 1, t->mis_list, s));
+        // Fallback when more particles' P2G cells differ from their sort
+        // cells than the gather walks.
         for (int a = 0; a < 3; ++a) {
-            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->num[a], 0u, P.F[a], t->counters + 1, s));
-            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, s));
+            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->num[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
+            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
         }
-        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, s));
-        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, s));
+        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
+        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
         t->gather_last = 1;
     } else {
         for (int a = 0; a < 3; ++a) {
             FS_HIP(launch_fill_u32((uint32_t*)t->num[a], 0u, P.F[a], s));
             FS_HIP(launch_fill_u32((uint32_t*)t->den[a], 0u, P.F[a], s));
         }
-        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, s));
-        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, nullptr, s));
+        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, 0u, s));
+        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, nullptr, 0u, s));
         t->gather_last = 0;
     }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[1], s));
@@ -538,26 +572,8 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
     if (!prm || !out || (n > 0 && !pos_xyz)) return fail(FS_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     const fs_params& q = *prm;
-    if (q.nx < 3 || q.ny < 3 || q.nz < 3) return fail(FS_ERR_INVALID_ARGUMENT, "grid must be at least 3 cells per axis");
     const uint64_t C = (uint64_t)q.nx * q.ny * q.nz;
-    if ((uint64_t)(q.nx + 1) * (q.ny + 1) * (q.nz + 1) >= (1ull << 31))
-        return fail(FS_ERR_INVALID_ARGUMENT, "grid too large for 32-bit face indices");
-    if (!(q.h > 0.0f) || !std::isfinite(q.h) || !(q.dt > 0.0f) || !std::isfinite(q.dt) || !(q.rho > 0.0f))
-        return fail(FS_ERR_INVALID_ARGUMENT, "h, dt and rho must be positive and finite");
-    const float scalars[] = {q.origin[0], q.origin[1], q.origin[2], q.gravity[0], q.gravity[1], q.gravity[2],
-                             q.flip_blend, q.wall_friction, q.rest_density, q.volume_stiffness, q.drift_clamp,
-                             q.cfl, q.pressure_residual_tolerance, q.rho};
-    if (!all_finite(scalars, sizeof scalars / sizeof scalars[0]))
-        return fail(FS_ERR_NOT_FINITE, "non-finite parameter");
-    if (q.engine > FS_ENGINE_FUSED || q.precision > FS_PRECISION_MIXED || q.velocity_storage > FS_VELOCITY_BF16)
-        return fail(FS_ERR_INVALID_ARGUMENT, "unknown engine/precision/velocity_storage");
-    if (q.engine == FS_ENGINE_REFERENCE && q.precision != FS_PRECISION_F32)
-        return fail(FS_ERR_UNSUPPORTED, "the reference engine runs FS_PRECISION_F32 only");
-    if (q.engine == FS_ENGINE_FUSED && q.precision != FS_PRECISION_MIXED)
-        return fail(FS_ERR_UNSUPPORTED, "the fused engine runs FS_PRECISION_MIXED only");
-    if (n > 0 && (!all_finite(pos_xyz, 3 * (size_t)n) || !all_finite(vel_xyz, vel_xyz ? 3 * (size_t)n : 0)))
-        return fail(FS_ERR_NOT_FINITE, "non-finite particle position or velocity");
-    if ((uint64_t)n > (1ull << 31)) return fail(FS_ERR_INVALID_ARGUMENT, "too many particles");
+    if (int rc0 = check_params(q, n, pos_xyz, vel_xyz); rc0 != FS_OK) return rc0;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FS_ERR_DEVICE, "no HIP device available");
 
@@ -608,15 +624,12 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(t->alloc(t->scal, 8));
         FS_HIP(t->alloc(t->partials, (C + 255) / 256));
         FS_HIP(t->alloc(t->counters, 4));
+        FS_HIP(t->alloc(t->mis_list, kMisfitCap));
         const size_t ntiles = C / 64 + 1;
         t->cg_blocks = cg_persistent_max_blocks();
         if (t->cg_blocks <= 0) return fail(FS_ERR_DEVICE, "cannot size the persistent CG grid");
         const int max_blocks = t->cg_blocks;
-        // The occupancy figure is not a residency guarantee: on MI355X a grid
-        // of the full figure (8 blocks of 256 per CU) did not become resident
-        // at once and its grid barrier timed out, while half of it did.  Use
-        // half, in whole XCD groups (multiple of 8 blocks).
-        t->cg_blocks = (t->cg_blocks / 2 / 8) * 8;
+        t->cg_blocks = half_resident_blocks(t->cg_blocks);
         if (const char* env = std::getenv("FLUIDSTEP_CG_BLOCKS")) {
             int v = std::atoi(env);
             if (v > 0 && v <= max_blocks) t->cg_blocks = v;
@@ -891,7 +904,7 @@ FS_API int fs_get_stats(fs_tank* t, fs_stats* out) {
     FS_HIP(hipMemcpy(cnt, t->counters, sizeof cnt, hipMemcpyDeviceToHost));
     s.liquid_cells = cnt[0];
     s.sorted = (uint32_t)t->sorted_last;
-    s.p2g_gather = (uint32_t)(t->gather_last && cnt[1] == 0);
+    s.p2g_gather = (uint32_t)(t->gather_last && cnt[1] <= kMisfitCap);
     if (t->prm.engine == FS_ENGINE_FUSED) {
         CgState st;
         FS_HIP(hipMemcpy(&st, t->cg.state, sizeof st, hipMemcpyDeviceToHost));

@@ -13,18 +13,24 @@ struct ParticleSoA {
     void* vel[3];
 };
 
+// Listed misfit particles (P2G cell != sort cell) the brick gather walks;
+// beyond this many the step takes the atomic P2G form instead.
+constexpr uint32_t kMisfitCap = 4096;
+
 // ---- k_bin.hip: clears, keys, histogram, scans, stable radix permutation ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float value, size_t count, hipStream_t s);
 // sh/mark.wgsl as written: one global atomic per particle.
 hipError_t launch_mark(const StepParams& P, const float* const pos[3], uint32_t* occ, hipStream_t s);
 // Keys + histogram with wave-aggregated atomics; keys[p] = cell_key(p).
-// misfit counts particles whose P2G cell (floor((pos-origin)*inv_h)) differs
-// from their sort cell; the brick gather needs it to be zero.
-hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys,
-                           uint32_t* occ, uint32_t* misfit, hipStream_t s);
+// mis_count counts particles whose P2G cell (floor((pos-origin)*inv_h))
+// differs from their sort cell; the first mis_cap of them are listed in
+// mis_list (nullable) for the brick gather.
+hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys, uint32_t* occ,
+                           uint32_t* mis_count, uint32_t* mis_list, uint32_t mis_cap, hipStream_t s);
+// Fill that runs only when *guard > above.
 hipError_t launch_fill_u32_guarded(uint32_t* p, uint32_t value, size_t count, const uint32_t* guard,
-                                   hipStream_t s);
+                                   uint32_t above, hipStream_t s);
 // Exclusive scan workspace size (u32 elements) for `len` inputs.
 size_t scan_workspace(size_t len);
 hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
@@ -32,9 +38,10 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, 
 size_t radix_workspace(size_t n);  // u32 elements (histogram table + scan ws)
 // Stable LSD radix sort of keys[0..n) (keys < key_limit); writes perm[d] = the
 // source index of sorted slot d.  keys_tmp/vals_tmp are n-sized scratch.
+// tile_variant picks the keys per workgroup: 0 = 4096, 1 = 2048, 2 = 8192.
 hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t key_limit,
                                      uint32_t* keys_tmp0, uint32_t* keys_tmp1, uint32_t* vals_tmp,
-                                     uint32_t* perm, uint32_t* ws, hipStream_t s);
+                                     uint32_t* perm, uint32_t* ws, hipStream_t s, int tile_variant = 0);
 // dst[d] = src[perm[d]] for the six particle columns.
 hipError_t launch_permute(uint32_t n, const uint32_t* perm, const ParticleSoA& src,
                           const ParticleSoA& dst, int bf16, hipStream_t s);
@@ -43,10 +50,10 @@ hipError_t launch_permute(uint32_t n, const uint32_t* perm, const ParticleSoA& s
 hipError_t launch_classify(const StepParams& P, const uint32_t* occ, uint8_t* ct, uint32_t* stats,
                            hipStream_t s);
 // sh/scatter.wgsl: per-particle i32 atomics into num/den.
-// guard (nullable): run only when *guard != 0.
+// guard (nullable): run only when *guard > above.
 hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf16,
                              int32_t* const num[3], int32_t* const den[3], const uint32_t* guard,
-                             hipStream_t s);
+                             uint32_t above, hipStream_t s);
 // Reference-structured per-axis passes (one kernel each, sh/*.wgsl).
 hipError_t launch_normalize(const StepParams& P, int a, const int32_t* num, const int32_t* den,
                             float* vel, hipStream_t s);
@@ -60,15 +67,19 @@ hipError_t launch_divergence(const StepParams& P, float* const vel[3], const uin
 // Fused: normalize -> save -> forces -> enforce for all three axes.
 hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3],
                                 const int32_t* const den[3], const uint8_t* ct, float* const vel[3],
-                                float* const saved[3], const uint32_t* guard, hipStream_t s);
+                                float* const saved[3], const uint32_t* guard, uint32_t above,
+                                hipStream_t s);
 // Fused: gradient -> enforce for all three axes.
 hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const uint8_t* ct,
                                    float* const vel[3], hipStream_t s);
 // Sorted-segment P2G gather per brick of cells, with the face epilogue fused.
-hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, int bf16,
+// brick_variant: 0 = 16x8x8, 1 = 8x8x8, 2 = 32x4x4 cells per workgroup.
+// src is the pre-sort side, which the listed misfit indices refer to; the
+// gather does nothing when *mis_count exceeds kMisfitCap (fs_gather.h).
+hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
                              const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct,
-                             float* const vel[3], float* const saved[3], const uint32_t* misfit,
-                             hipStream_t s);
+                             float* const vel[3], float* const saved[3], const uint32_t* mis_count,
+                             const uint32_t* mis_list, hipStream_t s, int brick_variant = 0);
 
 // ---- k_cg.hip: pressure solve ----
 // Reference-structured CG passes (sh/cg_*.wgsl), f32 scalars in `scal`.
@@ -114,6 +125,9 @@ struct CgWork {
     uint32_t* n_tiles;       // [1]
     CgState* state;          // [1]
 };
+// tiles[scan[t]] = t for flagged tiles; *n_active = the flagged count.
+hipError_t launch_compact_tiles(uint32_t ntiles, const uint32_t* flags, const uint32_t* scan, uint32_t* tiles,
+                                uint32_t* n_active, hipStream_t s);
 hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uint8_t* ct,
                              const uint32_t* occ, float* div, float* b, float* p, float* r, float* d,
                              const CgWork& w, uint32_t* scan_ws, hipStream_t s);

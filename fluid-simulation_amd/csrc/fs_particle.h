@@ -1,0 +1,189 @@
+// fs_particle.h — per-particle device functions (G2P and atomic P2G) shared
+// by the single-tank and the batched kernels.
+#pragma once
+#include "fs_face.h"
+#include "fs_kernels.h"
+
+namespace fs {
+
+// sample_u/v/w of sh/g2p.wgsl:56-129: wall-aware trilinear gather of
+// (final, saved) with renormalized weights.
+__device__ __forceinline__ void sample_axis(const StepParams& P, int a, const float Pp[3],
+                                            const float* __restrict__ vel, const float* __restrict__ sav,
+                                            float& ofin, float& osav) {
+    uint32_t dim[3];
+    face_dims(P, a, dim);
+    float g[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[c] = (Pp[c] - P.origin[c]) * P.inv_h + stagger(a, c);
+    const int32_t bx = sat_i32(floorf(g[0])), by = sat_i32(floorf(g[1])), bz = sat_i32(floorf(g[2]));
+    const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
+    const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
+    const int32_t lx = a == 0 ? 1 : 0, ly = a == 1 ? 1 : 0, lz = a == 2 ? 1 : 0;
+    float fin = 0.0f, sv = 0.0f, wsum = 0.0f;
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk) {
+        const int32_t kk = bz + dk;
+        if (kk < 0 || kk >= (int32_t)dim[2]) continue;
+        const float wz = dk == 1 ? tz : 1.0f - tz;
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const int32_t jj = by + dj;
+            if (jj < 0 || jj >= (int32_t)dim[1]) continue;
+            const float wy = dj == 1 ? ty : 1.0f - ty;
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int32_t ii = bx + di;
+                if (ii < 0 || ii >= (int32_t)dim[0]) continue;
+                // *_face_touches_static_solid (sh/g2p.wgsl:35-54)
+                if (ii <= lx || ii >= nx - 1 || jj <= ly || jj >= ny - 1 || kk <= lz || kk >= nz - 1) continue;
+                const float wx = di == 1 ? tx : 1.0f - tx;
+                const float w = wx * wy * wz;
+                const uint32_t idx = (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk);
+                fin += w * vel[idx];
+                sv += w * sav[idx];
+                wsum += w;
+            }
+        }
+    }
+    if (wsum > 0.0f) {
+        ofin = fin / wsum;
+        osav = sv / wsum;
+    } else {
+        ofin = 0.0f;
+        osav = 0.0f;
+    }
+}
+
+// G2P + FLIP/PIC blend + CFL clamp + wall friction + advection + wall clamp
+// for particle p (sh/g2p.wgsl:131-198), in place.
+template <int BF16>
+__device__ __forceinline__ void g2p_particle(const StepParams& P, const ParticleSoA& sp, uint32_t p,
+                                             const float* __restrict__ u, const float* __restrict__ v,
+                                             const float* __restrict__ w, const float* __restrict__ us,
+                                             const float* __restrict__ vs, const float* __restrict__ ws) {
+    const float h = P.h, dt = P.dt, alpha = P.flip;
+    const float Pp[3] = {sp.pos[0][p], sp.pos[1][p], sp.pos[2][p]};
+    const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p), load_vel<BF16>(sp.vel[2], p)};
+    float pic[3], sav[3];
+    sample_axis(P, 0, Pp, u, us, pic[0], sav[0]);
+    sample_axis(P, 1, Pp, v, vs, pic[1], sav[1]);
+    sample_axis(P, 2, Pp, w, ws, pic[2], sav[2]);
+    float nv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float flip = pv[a] + (pic[a] - sav[a]);
+        nv[a] = pic[a] * (1.0f - alpha) + flip * alpha;  // WGSL mix(pic, flip, alpha)
+    }
+    const float cfl = fmaxf(P.cfl, 1.0f);
+    const float maxs = cfl * h / dt;
+    const float sp2 = sqrtf(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    if (sp2 > maxs && sp2 > 0.0f) {
+        const float s = maxs / sp2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) nv[a] = nv[a] * s;
+    }
+    const float hband = h * 1.05f;
+    const float ext[3] = {(float)P.nx * h, (float)P.ny * h, (float)P.nz * h};
+    float lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = P.origin[a] + hband;
+        hi[a] = P.origin[a] + ext[a] - hband;
+    }
+    const float friction = P.friction;
+    if (friction > 0.0f) {
+        const float band = h * 2.5f;
+        float nr[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float dlo = (Pp[a] - lo[a]) / band;
+            const float dhi = (hi[a] - Pp[a]) / band;
+            nr[a] = clampf(1.0f - fminf(dlo, dhi), 0.0f, 1.0f);
+        }
+        nv[0] = nv[0] * (1.0f - friction * fmaxf(nr[1], nr[2]));
+        nv[1] = nv[1] * (1.0f - friction * fmaxf(nr[0], nr[2]));
+        nv[2] = nv[2] * (1.0f - friction * fmaxf(nr[0], nr[1]));
+    }
+    float np[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        np[a] = Pp[a] + nv[a] * dt;
+        if (np[a] < lo[a]) { np[a] = lo[a]; nv[a] = 0.0f; }
+        if (np[a] > hi[a]) { np[a] = hi[a]; nv[a] = 0.0f; }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sp.pos[a][p] = np[a];
+        store_vel<BF16>(sp.vel[a], p, nv[a]);
+    }
+}
+
+// One particle of sh/scatter.wgsl:78-118 with global i32 atomics.
+template <int BF16>
+__device__ __forceinline__ void p2g_atomic_particle(const StepParams& P, const ParticleSoA& sp, uint32_t p,
+                                                    int32_t* const nums[3], int32_t* const dens[3]) {
+    const float pos[3] = {(sp.pos[0][p] - P.origin[0]) * P.inv_h, (sp.pos[1][p] - P.origin[1]) * P.inv_h,
+                          (sp.pos[2][p] - P.origin[2]) * P.inv_h};
+    const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p), load_vel<BF16>(sp.vel[2], p)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        uint32_t dim[3];
+        face_dims(P, a, dim);
+        const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
+        int32_t* nm = nums[a];
+        int32_t* dn = dens[a];
+        p2g_taps(g, dim, pv[a], P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
+            const uint32_t idx = (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk);
+            atomicAdd(&nm[idx], tn);
+            atomicAdd(&dn[idx], td);
+        });
+    }
+}
+
+// Sort key of particle p plus whether its P2G cell differs from it.
+__device__ __forceinline__ uint32_t particle_key(const StepParams& P, float x, float y, float z, bool& misfit) {
+    const uint32_t key = cell_key(P, x, y, z);
+    const uint32_t ki = key % P.nx, kj = (key / P.nx) % P.ny, kk = key / (P.nx * P.ny);
+    misfit = floorf((x - P.origin[0]) * P.inv_h) != (float)ki || floorf((y - P.origin[1]) * P.inv_h) != (float)kj ||
+             floorf((z - P.origin[2]) * P.inv_h) != (float)kk;
+    return key;
+}
+
+// Wave-level list append of the lanes with mis set: every such lane's index p
+// goes to list[slot] for slot < cap; *count (and *tank_count, nullable) grow
+// by the number of such lanes.  All lanes of the wave must call it.
+__device__ __forceinline__ void list_misfits(bool mis, uint32_t p, uint32_t* count, uint32_t* list, uint32_t cap,
+                                             uint32_t* tank_count) {
+    const uint64_t mm = __ballot(mis);
+    if (!mm) return;
+    const uint32_t lane = lane_id();
+    uint32_t base = 0;
+    if (lane == 0) {
+        base = atomicAdd(count, (uint32_t)__popcll(mm));
+        if (tank_count) atomicAdd(tank_count, (uint32_t)__popcll(mm));
+    }
+    base = __shfl(base, 0, 64);
+    if (mis && list) {
+        const uint32_t slot = base + (uint32_t)__popcll(mm & lanemask_lt());
+        if (slot < cap) list[slot] = p;
+    }
+}
+
+// occ[key] += 1 for every valid lane, one atomic per run of equal keys among
+// the wave's lanes (sorted particles come in runs); the counts are integers,
+// so the result equals one atomic per particle.  All lanes must call it.
+__device__ __forceinline__ void hist_key_runs(uint32_t key, bool valid, uint32_t* occ) {
+    const uint32_t lane = lane_id();
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t act = __ballot(valid);
+    if (head) {
+        const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)(64 - __builtin_clzll(act));
+        atomicAdd(&occ[key], end - lane);
+    }
+}
+
+}  // namespace fs

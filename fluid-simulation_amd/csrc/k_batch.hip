@@ -1,0 +1,458 @@
+// k_batch.hip — the batched-tank step: every phase of the fused substep as one
+// launch over all tanks of a batch.  Each workgroup takes one WorkItem of its
+// phase's table (fs_batch.h), loads that tank's constants, offsets the
+// concatenated arrays to the tank and runs the same per-particle, per-cell,
+// per-face and per-brick device functions as the single-tank kernels
+// (fs_particle.h, fs_cell.h, fs_face.h, fs_gather.h, fs_cg.h), so a tank of a
+// batch computes what a single tank computes.
+//
+// The pressure solve is one cooperative launch.  Its workgroups are split into
+// one group per tank (sized on the host by the tank's expected liquid volume);
+// a group runs that tank's mixed-precision CG with its own arrival counter,
+// generation word and partial slots, so tanks never wait for each other and a
+// tank that converges early releases its workgroups.
+#include "fs_batch.h"
+#include "fs_cell.h"
+#include "fs_gather.h"
+
+namespace fs {
+
+namespace {
+constexpr int kBCgThreads = 512;   // CG workgroup (8 waves)
+constexpr int kBLdsTiles = 2048;   // tile ids a CG workgroup keeps in LDS
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Keys, histogram, misfit list (k_key_hist per tank; keys are global cells).
+__global__ __launch_bounds__(kBatchBlock) void kb_key_hist(const BatchTank* __restrict__ T,
+                                                           const WorkItem* __restrict__ wl, ParticleSoA sp,
+                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ occ,
+                                                           uint32_t* __restrict__ counters,
+                                                           uint32_t* __restrict__ mis_count,
+                                                           uint32_t* __restrict__ mis_list) {
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const StepParams P = T[wi.tank].P;
+    const uint32_t part_off = T[wi.tank].part_off, cell_off = T[wi.tank].cell_off;
+    const uint32_t local = wi.start + threadIdx.x;
+    const bool valid = local < P.N;
+    const uint32_t p = part_off + local;
+    uint32_t key = 0xffffffffu;
+    bool mis = false;
+    if (valid) {
+        key = cell_off + particle_key(P, sp.pos[0][p], sp.pos[1][p], sp.pos[2][p], mis);
+        keys[p] = key;
+    }
+    list_misfits(mis, p, mis_count, mis_list, kMisfitCap, &counters[4 * wi.tank + 1]);
+    hist_key_runs(key, valid, occ);
+}
+
+hipError_t launch_batch_key_hist(const BatchTank* T, const WorkItem* wl, uint32_t nwl, const ParticleSoA& sp,
+                                 uint32_t* keys, uint32_t* occ, uint32_t* counters, uint32_t* mis_count,
+                                 uint32_t* mis_list, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_key_hist, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, keys, occ, counters, mis_count,
+                       mis_list);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBatchBlock) void kb_classify(const BatchTank* __restrict__ T,
+                                                           const WorkItem* __restrict__ wl,
+                                                           const uint32_t* __restrict__ occ, uint8_t* __restrict__ ct,
+                                                           uint32_t* __restrict__ counters) {
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const StepParams P = T[wi.tank].P;
+    const uint32_t co = T[wi.tank].cell_off;
+    const uint32_t c = wi.start + threadIdx.x;
+    bool liquid = false;
+    if (c < P.C) {
+        const uint8_t t = classify_cell(P, occ + co, c);
+        ct[co + c] = t;
+        liquid = t == kLiquid;
+    }
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const uint64_t m = __ballot(liquid);
+    if (lane_id() == 0 && m) atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_cnt) atomicAdd(&counters[4 * wi.tank], s_cnt);
+}
+
+hipError_t launch_batch_classify(const BatchTank* T, const WorkItem* wl, uint32_t nwl, const uint32_t* occ,
+                                 uint8_t* ct, uint32_t* counters, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_classify, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, occ, ct, counters);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gather: work item = (tank, brick index of the tank's brick grid).
+template <int BF16, int V>
+__global__ __launch_bounds__(kGThreads) void kb_gather(const BatchTank* __restrict__ T,
+                                                        const WorkItem* __restrict__ wl, ParticleSoA sp,
+                                                        ParticleSoA src, const uint32_t* __restrict__ occ,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint8_t* __restrict__ ct, BatchGrid g,
+                                                        const uint32_t* __restrict__ mis_count,
+                                                        const uint32_t* __restrict__ mis_list) {
+    const uint32_t mis_n = *mis_count;
+    if (mis_n > kMisfitCap) return;  // the atomic form handles this substep
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const BatchTank& bt = T[wi.tank];
+    const StepParams P = bt.P;
+    const uint32_t co = bt.cell_off, part_off = bt.part_off;
+    const uint32_t nbx = (P.nx + Brick<V>::X - 1) / Brick<V>::X, nby = (P.ny + Brick<V>::Y - 1) / Brick<V>::Y;
+    const uint32_t b = wi.start;
+    const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
+    const GatherOut out = {{g.vel[0] + bt.face_off[0], g.vel[1] + bt.face_off[1], g.vel[2] + bt.face_off[2]},
+                           {g.saved[0] + bt.face_off[0], g.saved[1] + bt.face_off[1], g.saved[2] + bt.face_off[2]}};
+    p2g_gather_brick<BF16, V>(P, sp, src, occ + co, off + co, ct + co, out, (int32_t)bxi * Brick<V>::X,
+                              (int32_t)byi * Brick<V>::Y, (int32_t)bzi * Brick<V>::Z, mis_list, mis_n, part_off,
+                              part_off + P.N);
+}
+
+namespace {
+template <int V>
+hipError_t launch_batch_gather_v(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int bf16,
+                                 const ParticleSoA& sorted, const ParticleSoA& src, const uint32_t* occ,
+                                 const uint32_t* off, const uint8_t* ct, const BatchGrid& g,
+                                 const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s) {
+    if (bf16)
+        hipLaunchKernelGGL((kb_gather<1, V>), dim3(nwl), dim3(kGThreads), 0, s, T, wl, sorted, src, occ, off, ct, g,
+                           mis_count, mis_list);
+    else
+        hipLaunchKernelGGL((kb_gather<0, V>), dim3(nwl), dim3(kGThreads), 0, s, T, wl, sorted, src, occ, off, ct, g,
+                           mis_count, mis_list);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_batch_gather(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int brick_variant, int bf16,
+                               const ParticleSoA& sorted, const ParticleSoA& src, const uint32_t* occ,
+                               const uint32_t* cell_offset, const uint8_t* ct, const BatchGrid& g,
+                               const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    switch (brick_variant) {
+        case 0: return launch_batch_gather_v<0>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list, s);
+        case 1: return launch_batch_gather_v<1>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list, s);
+        case 2: return launch_batch_gather_v<2>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fallback: atomic P2G and the face epilogue, when too many misfits.
+template <int BF16>
+__global__ __launch_bounds__(kBatchBlock) void kb_p2g_atomic(const BatchTank* __restrict__ T,
+                                                             const WorkItem* __restrict__ wl, ParticleSoA sp,
+                                                             BatchGrid g, const uint32_t* __restrict__ mis_count) {
+    if (*mis_count <= kMisfitCap) return;
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const BatchTank& bt = T[wi.tank];
+    const StepParams P = bt.P;
+    const uint32_t local = wi.start + threadIdx.x;
+    if (local >= P.N) return;
+    int32_t* const nums[3] = {g.num[0] + bt.face_off[0], g.num[1] + bt.face_off[1], g.num[2] + bt.face_off[2]};
+    int32_t* const dens[3] = {g.den[0] + bt.face_off[0], g.den[1] + bt.face_off[1], g.den[2] + bt.face_off[2]};
+    p2g_atomic_particle<BF16>(P, sp, bt.part_off + local, nums, dens);
+}
+
+__global__ __launch_bounds__(kBatchBlock) void kb_face_epilogue(const BatchTank* __restrict__ T,
+                                                                const WorkItem* __restrict__ wl,
+                                                                const uint8_t* __restrict__ ct, BatchGrid g,
+                                                                const uint32_t* __restrict__ mis_count) {
+    if (*mis_count <= kMisfitCap) return;
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const BatchTank& bt = T[wi.tank];
+    const StepParams P = bt.P;
+    int a;
+    uint32_t f;
+    if (!split_face(P, wi.start + threadIdx.x, a, f)) return;
+    const uint32_t fo = bt.face_off[a];
+    uint32_t d[3], i, j, k;
+    face_dims(P, a, d);
+    face_ijk(f, d, i, j, k);
+    float vv, ss;
+    face_epilogue(P, a, ct + bt.cell_off, i, j, k, g.num[a][fo + f], g.den[a][fo + f], vv, ss);
+    g.vel[a][fo + f] = vv;
+    g.saved[a][fo + f] = ss;
+}
+
+hipError_t launch_batch_p2g_fallback(const BatchTank* T, const WorkItem* wl_part, uint32_t npart,
+                                     const WorkItem* wl_face, uint32_t nface, int bf16, const ParticleSoA& sorted,
+                                     const uint8_t* ct, const BatchGrid& g, const uint32_t* mis_count,
+                                     hipStream_t s) {
+    if (npart) {
+        if (bf16)
+            hipLaunchKernelGGL(kb_p2g_atomic<1>, dim3(npart), dim3(kBatchBlock), 0, s, T, wl_part, sorted, g, mis_count);
+        else
+            hipLaunchKernelGGL(kb_p2g_atomic<0>, dim3(npart), dim3(kBatchBlock), 0, s, T, wl_part, sorted, g, mis_count);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nface) hipLaunchKernelGGL(kb_face_epilogue, dim3(nface), dim3(kBatchBlock), 0, s, T, wl_face, ct, g, mis_count);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBatchBlock) void kb_div_cginit(const BatchTank* __restrict__ T,
+                                                             const WorkItem* __restrict__ wl, BatchGrid g,
+                                                             const uint8_t* __restrict__ ct,
+                                                             const uint32_t* __restrict__ occ, float* __restrict__ div,
+                                                             float* __restrict__ b, float* __restrict__ p,
+                                                             float* __restrict__ r, float* __restrict__ d,
+                                                             BatchCgWork w) {
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const BatchTank& bt = T[wi.tank];
+    const StepParams P = bt.P;
+    const uint32_t co = bt.cell_off;
+    const uint32_t c = wi.start + threadIdx.x;  // wi.start is a multiple of 256: waves are whole tiles
+    bool liquid = false;
+    if (c < P.C)
+        liquid = div_cginit_cell(P, g.vel[0] + bt.face_off[0], g.vel[1] + bt.face_off[1], g.vel[2] + bt.face_off[2],
+                                 ct + co, occ + co, c, div + co, b + co, p + co, r + co, d + co, w.nmask + co);
+    const uint64_t m = __ballot(liquid);
+    const uint32_t tile = c / kTile;
+    if (lane_id() == 0 && tile * kTile < P.C) w.tile_flags[bt.tile_off + tile] = m ? 1u : 0u;
+}
+
+hipError_t launch_batch_div_cginit(const BatchTank* T, const WorkItem* wl, uint32_t nwl, const BatchGrid& g,
+                                   const uint8_t* ct, const uint32_t* occ, float* div, float* b, float* p, float* r,
+                                   float* d, const BatchCgWork& w, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_div_cginit, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, g, ct, occ, div, b, p, r, d, w);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Group barrier + sum: like the single-tank grid_sum, over the workgroups of
+// one tank.  Arrival is a release RMW on the tank's counter; the last arrival
+// resets it and bumps the tank's generation word; the others poll the word
+// with relaxed loads and acquire once.  Every workgroup then sums the group's
+// published partials in slot order, so all hold the identical total.
+// Partials alternate between two halves by generation parity.
+template <int THREADS>
+__device__ bool group_sum(double v, const BatchCgWork& w, uint32_t tank, uint32_t base, uint32_t nb, uint32_t rank,
+                          double& out) {
+    __shared__ double s_w[THREADS / 64];
+    __shared__ double s_tree[THREADS];
+    __shared__ int s_abort;
+    __shared__ uint32_t s_gen;
+    const double bs = block_sum<THREADS>(v, s_w);
+    uint32_t* cnt = w.arrive + 32u * tank;
+    uint32_t* gen = w.gen + 32u * tank;
+    if (threadIdx.x == 0) {
+        const uint32_t g0 = ld_relaxed(gen);
+        w.partials[(size_t)(g0 & 1u) * gridDim.x + base + rank] = bs;
+        s_gen = g0;
+        s_abort = 0;
+        const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == nb - 1u) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const long long t0 = wall_clock64();
+            while (ld_relaxed(gen) == g0) {
+                if (ld_relaxed(w.abort_flag)) {
+                    s_abort = 1;
+                    break;
+                }
+                if (wall_clock64() - t0 > kSpinTimeoutTicks) {
+                    __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_abort = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    __syncthreads();
+    if (s_abort) return false;
+    out = sum_partials<THREADS>(w.partials + (size_t)(s_gen & 1u) * gridDim.x + base, nb, s_tree);
+    return true;
+}
+
+template <int U>
+__global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict__ T, const WorkItem* __restrict__ wl,
+                                                      const float* __restrict__ b, float* __restrict__ p,
+                                                      float* __restrict__ r, float* __restrict__ d0,
+                                                      float* __restrict__ d1, float* __restrict__ q, BatchCgWork w) {
+    __shared__ uint32_t s_tiles[kBLdsTiles];
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;  // padding workgroup: in no group
+    const uint32_t tank = wi.tank;
+    const BatchTank& bt = T[tank];
+    const StepParams P = bt.P;
+    const uint32_t co = bt.cell_off, tile_off = bt.tile_off;
+    const uint32_t G = bt.cg_nb, rank = wi.start, base = bt.cg_base;
+    // The tank's liquid tiles are the slice [tb, te) of the compacted list.
+    const uint32_t tb = w.tile_scan[tile_off], te = w.tile_scan[tile_off + bt.ntiles];
+    const uint32_t nt = te - tb;
+    const TileWalk tw = tile_walk(nt, rank, G, kBCgThreads / 64);
+    const uint32_t s0 = (uint32_t)(((uint64_t)nt * rank) / G);
+    const uint32_t s1 = (uint32_t)(((uint64_t)nt * (rank + 1u)) / G);
+    const bool in_lds = s1 - s0 <= (uint32_t)kBLdsTiles;
+    if (in_lds)
+        for (uint32_t i = s0 + threadIdx.x; i < s1; i += kBCgThreads) s_tiles[i - s0] = w.tiles[tb + i] - tile_off;
+    __syncthreads();
+    const uint32_t* gt = w.tiles + tb;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return in_lds ? s_tiles[t - s0] : gt[t] - tile_off; };
+    const uint32_t step = tw.stride * U;
+    const uint16_t* nm = w.nmask + co;
+    const float* bb = b + co;
+    float* pp = p + co;
+    float* rr_ = r + co;
+    float* q_ = q + co;
+
+    auto report = [&](double rs_init, double rs_final, double tr, uint32_t iters, uint32_t err) {
+        if (rank == 0 && threadIdx.x == 0) {
+            BatchCgState* st = w.state + tank;
+            st->rs_init = rs_init;
+            st->rs_final = rs_final;
+            st->true_res_sq = tr;
+            st->iterations = iters;
+            st->error = err;
+        }
+    };
+
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<U>(P, rr_, tw, ti, tile_id);
+    double rs0;
+    if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, rs0)) {
+        report(0.0, 0.0, 0.0, 0u, 1u);
+        return;
+    }
+    double rs_old = rs0;
+    const double rs_init = rs0;
+    const double tol = (double)P.tol;
+    const double tol_sq = tol * tol;
+    float beta = 0.0f;
+    float* dprev = d0 + co;
+    float* dnext = d1 + co;
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < P.iters; ++it) {
+        ++iters;
+        const bool first = it == 0;
+        float* dn_buf = first ? dprev : dnext;
+        acc = 0.0;
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+            acc += phase_a_batch<U>(P, nm, rr_, dprev, dnext, q_, beta, first, tw, ti, tile_id);
+        double dq;
+        if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, dq)) {
+            report(rs_init, rs_old, 0.0, iters, 1u);
+            return;
+        }
+        const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
+        acc = 0.0;
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+            acc += phase_b_batch<U>(P, dn_buf, q_, pp, rr_, alpha, tw, ti, tile_id);
+        double rr;
+        if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, rr)) {
+            report(rs_init, rs_old, 0.0, iters, 1u);
+            return;
+        }
+        beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
+        rs_old = rr;
+        if (tol > 0.0 && rr <= tol_sq * rs_init) break;
+        if (!first) {
+            float* t = dprev;
+            dprev = dnext;
+            dnext = t;
+        }
+    }
+    acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<U>(P, nm, bb, pp, tw, ti, tile_id);
+    double tr;
+    if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, tr)) {
+        report(rs_init, rs_old, 0.0, iters, 1u);
+        return;
+    }
+    report(rs_init, rs_old, tr, iters, 0u);
+}
+
+int batch_cg_max_blocks() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    int lo = 1 << 30;
+    // The smallest occupancy over the unroll variants, so every variant fits.
+    const void* fns[3] = {(const void*)kb_cg<4>, (const void*)kb_cg<2>, (const void*)kb_cg<8>};
+    for (const void* f : fns) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBCgThreads, 0) != hipSuccess) return 0;
+        lo = per_cu < lo ? per_cu : lo;
+    }
+    return cus * lo;
+}
+
+hipError_t launch_batch_cg(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int cg_unroll, const float* b,
+                           float* p, float* r, float* d0, float* d1, float* q, const BatchCgWork& w, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    void* args[] = {(void*)&T, (void*)&wl, (void*)&b, (void*)&p, (void*)&r,
+                    (void*)&d0, (void*)&d1, (void*)&q, (void*)&w};
+    const void* f = cg_unroll == 1 ? (const void*)kb_cg<2> : (cg_unroll == 2 ? (const void*)kb_cg<8> : (const void*)kb_cg<4>);
+    return hipLaunchCooperativeKernel(f, dim3(nwl), dim3(kBCgThreads), args, 0, s);
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBatchBlock) void kb_gradient_enforce(const BatchTank* __restrict__ T,
+                                                                   const WorkItem* __restrict__ wl,
+                                                                   const float* __restrict__ pr,
+                                                                   const uint8_t* __restrict__ ct, BatchGrid g) {
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const BatchTank& bt = T[wi.tank];
+    const StepParams P = bt.P;
+    int a;
+    uint32_t f;
+    if (!split_face(P, wi.start + threadIdx.x, a, f)) return;
+    float* vel = g.vel[a] + bt.face_off[a];
+    const uint8_t* ctt = ct + bt.cell_off;
+    uint32_t d[3], i, j, k;
+    face_dims(P, a, d);
+    face_ijk(f, d, i, j, k);
+    float v = face_gradient(P, a, pr + bt.cell_off, ctt, i, j, k, vel[f]);
+    if (face_solid(P, a, ctt, i, j, k)) v = 0.0f;
+    vel[f] = v;
+}
+
+hipError_t launch_batch_gradient_enforce(const BatchTank* T, const WorkItem* wl, uint32_t nwl, const float* p,
+                                         const uint8_t* ct, const BatchGrid& g, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_gradient_enforce, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, p, ct, g);
+    return hipGetLastError();
+}
+
+template <int BF16>
+__global__ __launch_bounds__(kBatchBlock) void kb_g2p(const BatchTank* __restrict__ T,
+                                                      const WorkItem* __restrict__ wl, ParticleSoA sp, BatchGrid g) {
+    const WorkItem wi = wl[blockIdx.x];
+    if (wi.tank == kNoTank) return;
+    const BatchTank& bt = T[wi.tank];
+    const StepParams P = bt.P;
+    const uint32_t local = wi.start + threadIdx.x;
+    if (local >= P.N) return;
+    const uint32_t* fo = bt.face_off;
+    g2p_particle<BF16>(P, sp, bt.part_off + local, g.vel[0] + fo[0], g.vel[1] + fo[1], g.vel[2] + fo[2],
+                       g.saved[0] + fo[0], g.saved[1] + fo[1], g.saved[2] + fo[2]);
+}
+
+hipError_t launch_batch_g2p(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int bf16, const ParticleSoA& sp,
+                            const BatchGrid& g, hipStream_t s) {
+    if (nwl == 0) return hipSuccess;
+    if (bf16) hipLaunchKernelGGL(kb_g2p<1>, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, g);
+    else hipLaunchKernelGGL(kb_g2p<0>, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, g);
+    return hipGetLastError();
+}
+
+}  // namespace fs

@@ -9,6 +9,7 @@
 // cell key (each 8-bit digit pass is stable), so the permutation is bit-exact
 // and deterministic.
 #include "fs_kernels.h"
+#include "fs_particle.h"
 
 namespace fs {
 
@@ -16,10 +17,12 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;     // 2048 elements per scan block
-constexpr int kRadixItems = 16;                    // keys per lane
 constexpr int kRadixWaves = kBlock / 64;
-constexpr int kRadixTile = kBlock * kRadixItems;   // 4096 keys per radix block
 constexpr int kRadixBins = 256;
+constexpr int kRadixMinTile = kBlock * 8;          // smallest menu entry (workspace sizing)
+// Keys per lane of the radix workgroup, by sort tile variant (0: 4096 keys,
+// 1: 2048, 2: 8192 per workgroup).
+constexpr int kRadixItemsMenu[3] = {16, 8, 32};
 
 inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 }  // namespace
@@ -36,20 +39,21 @@ __global__ void k_fill_f32(float* __restrict__ p, float v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// Fill that runs only when *guard != 0 (used for the P2G fallback path).
-__global__ void k_fill_u32_guarded(uint32_t* __restrict__ p, uint32_t v, size_t n, const uint32_t* guard) {
-    if (*guard == 0u) return;
+// Fill that runs only when *guard > above (used for the P2G fallback path).
+__global__ void k_fill_u32_guarded(uint32_t* __restrict__ p, uint32_t v, size_t n, const uint32_t* guard,
+                                   uint32_t above) {
+    if (*guard <= above) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
 
 hipError_t launch_fill_u32_guarded(uint32_t* p, uint32_t value, size_t count, const uint32_t* guard,
-                                   hipStream_t s) {
+                                   uint32_t above, hipStream_t s) {
     if (count == 0) return hipSuccess;
     unsigned g = grid_for(count, kBlock);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_fill_u32_guarded, dim3(g), dim3(kBlock), 0, s, p, value, count, guard);
+    hipLaunchKernelGGL(k_fill_u32_guarded, dim3(g), dim3(kBlock), 0, s, p, value, count, guard, above);
     return hipGetLastError();
 }
 
@@ -88,40 +92,28 @@ hipError_t launch_mark(const StepParams& P, const float* const pos[3], uint32_t*
 // integers, so the result equals one atomic per particle exactly.
 __global__ void k_key_hist(StepParams P, const float* __restrict__ px, const float* __restrict__ py,
                            const float* __restrict__ pz, uint32_t* __restrict__ keys,
-                           uint32_t* __restrict__ occ, uint32_t* __restrict__ misfit) {
+                           uint32_t* __restrict__ occ, uint32_t* __restrict__ mis_count,
+                           uint32_t* __restrict__ mis_list, uint32_t mis_cap) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < P.N;
     uint32_t key = 0xffffffffu;
     bool mis = false;
     if (valid) {
-        const float x = px[p], y = py[p], z = pz[p];
-        key = cell_key(P, x, y, z);
-        keys[p] = key;
         // The P2G cell (sh/scatter.wgsl:94: multiply by inv_h, no clamp) must be
-        // the sort cell for the brick gather to see every tap.
-        const uint32_t ki = key % P.nx, kj = (key / P.nx) % P.ny, kk = key / (P.nx * P.ny);
-        mis = floorf((x - P.origin[0]) * P.inv_h) != (float)ki || floorf((y - P.origin[1]) * P.inv_h) != (float)kj ||
-              floorf((z - P.origin[2]) * P.inv_h) != (float)kk;
+        // the sort cell for the brick gather to see every tap from the segments;
+        // particles where the two differ are counted and listed.
+        key = particle_key(P, px[p], py[p], pz[p], mis);
+        keys[p] = key;
     }
-    const uint64_t mm = __ballot(mis);
-    if (lane_id() == 0 && mm) atomicAdd(misfit, (uint32_t)__popcll(mm));
-    const uint32_t lane = lane_id();
-    uint32_t prev = __shfl_up(key, 1, 64);
-    const bool head = valid && (lane == 0 || prev != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t act = __ballot(valid);
-    if (head) {
-        uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-        uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)(64 - __builtin_clzll(act));
-        atomicAdd(&occ[key], end - lane);
-    }
+    list_misfits(mis, p, mis_count, mis_list, mis_cap, nullptr);
+    hist_key_runs(key, valid, occ);
 }
 
-hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys,
-                           uint32_t* occ, uint32_t* misfit, hipStream_t s) {
+hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys, uint32_t* occ,
+                           uint32_t* mis_count, uint32_t* mis_list, uint32_t mis_cap, hipStream_t s) {
     if (P.N == 0) return hipSuccess;
     hipLaunchKernelGGL(k_key_hist, dim3(grid_for(P.N, kBlock)), dim3(kBlock), 0, s, P, pos[0], pos[1],
-                       pos[2], keys, occ, misfit);
+                       pos[2], keys, occ, mis_count, mis_list, mis_cap);
     return hipGetLastError();
 }
 
@@ -220,15 +212,16 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, 
 
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort over 8-bit digits.
+template <int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                         uint32_t shift, uint32_t ntiles,
                                                         uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_h[kRadixBins];
     s_h[threadIdx.x] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kRadixTile;
+    const size_t base = (size_t)blockIdx.x * (ITEMS * kBlock);
 #pragma unroll 4
-    for (int i = 0; i < kRadixItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         size_t idx = base + threadIdx.x + (size_t)i * kBlock;
         if (idx < n) atomicAdd(&s_h[(keys[idx] >> shift) & 255u], 1u);
     }
@@ -236,9 +229,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
     hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// Each wave ranks 1024 consecutive keys in 16 rounds of 64; equal digits inside
-// a round are found with eight ballots.  Order of items = (wave, round, lane)
-// = input order, so equal digits keep their input order.
+// Each wave ranks 64*ITEMS consecutive keys in ITEMS rounds of 64; equal
+// digits inside a round are found with eight ballots.  Order of items =
+// (wave, round, lane) = input order, so equal digits keep their input order.
+template <int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
@@ -249,10 +243,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    const size_t base = (size_t)blockIdx.x * kRadixTile + (size_t)w * (kRadixTile / kRadixWaves);
-    uint32_t kk[kRadixItems], vv[kRadixItems], rk[kRadixItems];
+    const size_t base = (size_t)blockIdx.x * (ITEMS * kBlock) + (size_t)w * (ITEMS * 64);
+    uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
 #pragma unroll
-    for (int r = 0; r < kRadixItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         size_t idx = base + (size_t)r * 64 + lane;
         bool valid = idx < n;
         uint32_t key = valid ? keys_in[idx] : 0u;
@@ -287,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kRadixItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         if (rk[r] == 0xffffffffu) continue;
         uint32_t pos = s_cnt[w][(kk[r] >> shift) & 255u] + rk[r];
         if (write_keys) keys_out[pos] = kk[r];
@@ -296,19 +290,22 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 }
 
 size_t radix_workspace(size_t n) {
-    size_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    size_t ntiles = (n + kRadixMinTile - 1) / kRadixMinTile;
     size_t table = ntiles * kRadixBins;
     return 2 * table + scan_workspace(table) + 16;
 }
 
 hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t key_limit,
                                      uint32_t* keys_tmp0, uint32_t* keys_tmp1, uint32_t* vals_tmp,
-                                     uint32_t* perm, uint32_t* ws, hipStream_t s) {
+                                     uint32_t* perm, uint32_t* ws, hipStream_t s, int tile_variant) {
     if (n == 0) return hipSuccess;
+    if (tile_variant < 0 || tile_variant > 2) return hipErrorInvalidValue;
+    const int items = kRadixItemsMenu[tile_variant];
+    const uint32_t tile = (uint32_t)(items * kBlock);
     uint32_t bits = 1;
     while (bits < 32 && (1ull << bits) < (uint64_t)key_limit) ++bits;
     const uint32_t passes = (bits + 7) / 8;
-    const uint32_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    const uint32_t ntiles = (n + tile - 1) / tile;
     const size_t table = (size_t)ntiles * kRadixBins;
     uint32_t* hist = ws;
     uint32_t* offs = ws + table;
@@ -322,14 +319,26 @@ hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t 
     else { vbuf[0] = vals_tmp; vbuf[1] = perm; }
     for (uint32_t ps = 0; ps < passes; ++ps) {
         const uint32_t shift = 8 * ps;
-        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+        if (items == 16)
+            hipLaunchKernelGGL(k_radix_hist<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+        else if (items == 8)
+            hipLaunchKernelGGL(k_radix_hist<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+        else
+            hipLaunchKernelGGL(k_radix_hist<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         hipError_t e = launch_exclusive_scan(hist, offs, table, scan_ws, s);
         if (e != hipSuccess) return e;
         uint32_t* kout = kbuf[ps & 1];
         uint32_t* vout = vbuf[ps & 1];
         const int write_keys = ps + 1 < passes;
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
-                           n, shift, ntiles, write_keys);
+        if (items == 16)
+            hipLaunchKernelGGL(k_radix_scatter<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+                               n, shift, ntiles, write_keys);
+        else if (items == 8)
+            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+                               n, shift, ntiles, write_keys);
+        else
+            hipLaunchKernelGGL(k_radix_scatter<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+                               n, shift, ntiles, write_keys);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         kin = kout;

@@ -20,43 +20,17 @@
 //    src/sim/pressure.rs:282,296 do; the stop test rs_new <= tol^2 rs_init is
 //    f64 (src/sim/pressure.rs:262-292).
 //  * the f64 true residual |b - A p|^2 is reduced after the loop.
-#include "fs_kernels.h"
-#include "fs_face.h"
+#include "fs_cell.h"
+#include "fs_cg.h"
 
 namespace fs {
 
 namespace {
 inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
-constexpr int kTile = 64;             // cells per CG tile (one wave)
 constexpr int kCgThreads = 512;       // persistent workgroup (8 waves)
 constexpr int kCgMultiThreads = 256;  // multi-kernel workgroup
-constexpr int kU = 4;                 // tiles a wave loads before computing (memory-level parallelism)
 constexpr int kLdsTiles = 2048;       // tile ids a persistent block keeps in LDS
-constexpr long long kSpinTimeoutTicks = 400000000LL;  // wall_clock64 runs at 100 MHz
-// Stencil mask of a liquid cell: bits 0-2 non-solid neighbour count, bits 3-8
-// liquid neighbour in the order -x,+x,-y,+y,-z,+z, bit 9 the cell is liquid.
-constexpr uint16_t kMaskLiquid = 1u << 9;
 }  // namespace
-
-__device__ __forceinline__ float cell_div(const StepParams& P, const float* __restrict__ u,
-                                          const float* __restrict__ v, const float* __restrict__ w,
-                                          const uint32_t* __restrict__ occ, uint32_t c) {
-    const uint32_t nx = P.nx, ny = P.ny;
-    const uint32_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-    const float u_hi = u[(i + 1u) + (nx + 1u) * (j + ny * k)];
-    const float u_lo = u[i + (nx + 1u) * (j + ny * k)];
-    const float v_hi = v[i + nx * ((j + 1u) + (ny + 1u) * k)];
-    const float v_lo = v[i + nx * (j + (ny + 1u) * k)];
-    const float w_hi = w[i + nx * (j + ny * (k + 1u))];
-    const float w_lo = w[i + nx * (j + ny * k)];
-    float div = ((u_hi - u_lo) + (v_hi - v_lo) + (w_hi - w_lo)) * P.inv_h;
-    if (P.stiff > 0.0f && P.rest > 0.0f) {
-        const float over = fmaxf(0.0f, (float)occ[c] - P.rest) / P.rest;
-        const float drift = fminf(P.stiff * over, P.drift);
-        div = div - drift;
-    }
-    return div;
-}
 
 __global__ void k_div_cginit(StepParams P, const float* __restrict__ u, const float* __restrict__ v,
                              const float* __restrict__ w, const uint8_t* __restrict__ ct,
@@ -65,48 +39,7 @@ __global__ void k_div_cginit(StepParams P, const float* __restrict__ u, const fl
                              uint16_t* __restrict__ nmask, uint32_t* __restrict__ tile_flags) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     bool liquid = false;
-    if (c < P.C) {
-        liquid = ct[c] == kLiquid;
-        if (liquid) {
-            const uint32_t nx = P.nx, nxny = P.nx * P.ny;
-            const uint32_t nb[6] = {c - 1u, c + 1u, c - nx, c + nx, c - nxny, c + nxny};
-            uint16_t m = kMaskLiquid;
-            uint32_t cnt = 0;
-            float fcnt = 0.0f, sum = 0.0f;
-#pragma unroll
-            for (int s = 0; s < 6; ++s) {
-                const uint8_t t = ct[nb[s]];
-                if (t != kSolid) {
-                    ++cnt;
-                    fcnt += 1.0f;
-                    if (t == kLiquid) {
-                        m |= (uint16_t)(1u << (3 + s));
-                        if (P.warm) sum += p[nb[s]];
-                    }
-                }
-            }
-            m |= (uint16_t)cnt;
-            const float dv = cell_div(P, u, v, w, occ, c);
-            const float scale = P.rho * P.h * P.h / P.dt;
-            const float rhs = -scale * dv;
-            float ap = 0.0f;
-            if (P.warm) ap = fcnt * p[c] - sum;  // sh/cg_init.wgsl:24-48
-            else p[c] = 0.0f;
-            const float rc = rhs - ap;
-            div[c] = dv;
-            b[c] = rhs;
-            r[c] = rc;
-            d[c] = rc;
-            nmask[c] = m;
-        } else {
-            div[c] = 0.0f;
-            b[c] = 0.0f;
-            p[c] = 0.0f;
-            r[c] = 0.0f;
-            d[c] = 0.0f;
-            nmask[c] = 0;
-        }
-    }
+    if (c < P.C) liquid = div_cginit_cell(P, u, v, w, ct, occ, c, div, b, p, r, d, nmask);
     const uint64_t m = __ballot(liquid);
     const uint32_t tile = c / kTile;
     if (lane_id() == 0 && (size_t)tile * kTile < P.C) tile_flags[tile] = m ? 1u : 0u;
@@ -118,6 +51,13 @@ __global__ void k_compact_tiles(uint32_t ntiles, const uint32_t* __restrict__ fl
     if (t >= ntiles) return;
     if (flags[t]) tiles[scan[t]] = t;
     if (t == ntiles - 1u) *n_active = scan[t] + flags[t];
+}
+
+hipError_t launch_compact_tiles(uint32_t ntiles, const uint32_t* flags, const uint32_t* scan, uint32_t* tiles,
+                                uint32_t* n_active, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, ntiles, flags, scan, tiles,
+                       n_active);
+    return hipGetLastError();
 }
 
 hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uint8_t* ct, const uint32_t* occ,
@@ -148,36 +88,6 @@ hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uin
 // release on its counter writes the XCD's L2 back; the consumer's acquire
 // invalidates its caches once.  Waiting blocks poll with relaxed loads (an
 // acquire per poll would invalidate the shared L2 on every spin).
-__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int THREADS>
-__device__ __forceinline__ double block_sum(double v, double* s_w) {
-    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-    v = wave_sum(v);
-    if (lane == 0) s_w[wv] = v;
-    __syncthreads();
-    double bs = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < THREADS / 64; ++i) bs += s_w[i];
-    return bs;
-}
-
-template <int THREADS>
-__device__ __forceinline__ double sum_partials(const double* partials, uint32_t n, double* s_tree) {
-    double acc = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += THREADS)
-        acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_tree[threadIdx.x] = acc;
-    __syncthreads();
-    for (int d = THREADS / 2; d >= 1; d >>= 1) {
-        if ((int)threadIdx.x < d) s_tree[threadIdx.x] += s_tree[threadIdx.x + d];
-        __syncthreads();
-    }
-    return s_tree[0];
-}
-
 // Two-level arrival (thread 0 only); returns true for the block arriving last.
 // Arrivals are release RMWs; only a block that turns out to be a group's or
 // the grid's last arrival issues the acquire fence (fence-after-RMW
@@ -263,147 +173,6 @@ __device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
     return true;
 }
 
-// ---------------------------------------------------------------------------
-// Per-cell work, kU tiles at a time: all loads of the batch are issued before
-// any result is computed, so a wave keeps several memory round trips in flight.
-struct TileWalk {
-    uint32_t begin, end, stride;
-};
-
-__device__ __forceinline__ TileWalk tile_walk(uint32_t nt, uint32_t logical_block, uint32_t G, uint32_t waves) {
-    TileWalk t;
-    const uint32_t b0 = (uint32_t)(((uint64_t)nt * logical_block) / G);
-    t.begin = b0 + (threadIdx.x >> 6);
-    t.end = (uint32_t)(((uint64_t)nt * (logical_block + 1u)) / G);
-    t.stride = waves;
-    return t;
-}
-
-// Phase A for tiles ti, ti+stride, ... (kU of them): dn = first ? d : r + beta*d_prev,
-// q = A dn (liquid cells; 0 elsewhere); returns this lane's dn.q.
-template <typename TileId>
-__device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
-                                                const float* __restrict__ r, const float* __restrict__ dprev,
-                                                float* __restrict__ dnext, float* __restrict__ q, float beta,
-                                                bool first, const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
-    const uint32_t lane = lane_id(), C = P.C;
-    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
-    uint32_t c[kU];
-    uint16_t m[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const uint32_t t = ti + (uint32_t)u * tw.stride;
-        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
-        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
-    }
-    float rc[kU], dc[kU], rn[kU][6], dnb[kU][6];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const bool ok = c[u] < C;
-        rc[u] = (ok && !first) ? r[c[u]] : 0.0f;
-        dc[u] = ok ? dprev[c[u]] : 0.0f;
-#pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            const bool lq = (m[u] >> (3 + s)) & 1u;
-            const uint32_t nbc = (uint32_t)((int32_t)c[u] + off[s]);
-            rn[u][s] = (lq && !first) ? r[nbc] : 0.0f;
-            dnb[u][s] = lq ? dprev[nbc] : 0.0f;
-        }
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        if (c[u] >= C) continue;
-        const float dn = first ? dc[u] : rc[u] + beta * dc[u];
-        float qc = 0.0f;
-        if (m[u] & kMaskLiquid) {
-            float sum = 0.0f;
-#pragma unroll
-            for (int s = 0; s < 6; ++s)
-                if ((m[u] >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
-            qc = (float)(m[u] & 7u) * dn - sum;
-        }
-        if (!first) dnext[c[u]] = dn;
-        q[c[u]] = qc;
-        acc += (double)dn * (double)qc;
-    }
-    return acc;
-}
-
-template <typename TileId>
-__device__ __forceinline__ double phase_b_batch(const StepParams& P, const float* __restrict__ dn,
-                                                const float* __restrict__ q, float* __restrict__ p,
-                                                float* __restrict__ r, float alpha, const TileWalk& tw, uint32_t ti,
-                                                TileId&& tile_id) {
-    const uint32_t lane = lane_id(), C = P.C;
-    uint32_t c[kU];
-    float pv[kU], dv[kU], rv[kU], qv[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const uint32_t t = ti + (uint32_t)u * tw.stride;
-        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const bool ok = c[u] < C;
-        pv[u] = ok ? p[c[u]] : 0.0f;
-        dv[u] = ok ? dn[c[u]] : 0.0f;
-        rv[u] = ok ? r[c[u]] : 0.0f;
-        qv[u] = ok ? q[c[u]] : 0.0f;
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        if (c[u] >= C) continue;
-        p[c[u]] = pv[u] + alpha * dv[u];
-        const float rn = rv[u] - alpha * qv[u];
-        r[c[u]] = rn;
-        acc += (double)rn * (double)rn;
-    }
-    return acc;
-}
-
-template <typename TileId>
-__device__ __forceinline__ double rr_batch(const StepParams& P, const float* __restrict__ r, const TileWalk& tw,
-                                           uint32_t ti, TileId&& tile_id) {
-    const uint32_t lane = lane_id();
-    double acc = 0.0;
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const uint32_t t = ti + (uint32_t)u * tw.stride;
-        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : P.C;
-        if (c < P.C) {
-            const double rv = (double)r[c];
-            acc += rv * rv;
-        }
-    }
-    return acc;
-}
-
-template <typename TileId>
-__device__ __forceinline__ double true_res_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
-                                                 const float* __restrict__ b, const float* __restrict__ p,
-                                                 const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
-    const uint32_t lane = lane_id(), C = P.C;
-    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
-    double acc = 0.0;
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const uint32_t t = ti + (uint32_t)u * tw.stride;
-        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : C;
-        if (c >= C) continue;
-        const uint16_t m = nmask[c];
-        if (!(m & kMaskLiquid)) continue;
-        double sum = 0.0;
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-            if ((m >> (3 + s)) & 1u) sum += (double)p[(uint32_t)((int32_t)c + off[s])];
-        const double e = (double)b[c] - ((double)(m & 7u) * (double)p[c] - sum);
-        acc += e * e;
-    }
-    return acc;
-}
-
 // ---- persistent form ----
 __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, const uint8_t* __restrict__ ct,
                                                                const float* __restrict__ b, float* __restrict__ p,
@@ -428,7 +197,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
     const uint32_t step = tw.stride * kU;
 
     double acc = 0.0;
-    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch(P, r, tw, ti, tile_id);
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
     double rs0;
     if (!grid_sum<kCgThreads>(acc, w, rs0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
@@ -448,7 +217,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
         float* dn_buf = first ? dprev : dnext;
         acc = 0.0;
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-            acc += phase_a_batch(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
+            acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
         double dq;
         if (!grid_sum<kCgThreads>(acc, w, dq)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
@@ -457,7 +226,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
         const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
         acc = 0.0;
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-            acc += phase_b_batch(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
+            acc += phase_b_batch<kU>(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
         double rr;
         if (!grid_sum<kCgThreads>(acc, w, rr)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
@@ -473,7 +242,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
         }
     }
     acc = 0.0;
-    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch(P, w.nmask, b, p, tw, ti, tile_id);
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<kU>(P, w.nmask, b, p, tw, ti, tile_id);
     double tr;
     if (!grid_sum<kCgThreads>(acc, w, tr)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
@@ -514,7 +283,7 @@ __global__ __launch_bounds__(kCgMultiThreads) void k_cg_rs0(StepParams P, const 
     const uint32_t* gt = w.tiles;
     auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
     double acc = 0.0;
-    for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU) acc += rr_batch(P, r, tw, ti, tile_id);
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
     double tot;
     if (last_block_sum<kCgMultiThreads>(acc, w, tot) && threadIdx.x == 0) {
         CgState* st = w.state;
@@ -541,7 +310,7 @@ __global__ __launch_bounds__(kCgMultiThreads) void k_cg_phase_a(StepParams P, co
     auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
-        acc += phase_a_batch(P, w.nmask, r, dprev, dnext, q, beta, first != 0, tw, ti, tile_id);
+        acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first != 0, tw, ti, tile_id);
     double dq;
     if (last_block_sum<kCgMultiThreads>(acc, w, dq) && threadIdx.x == 0) {
         st->alpha = fabs(dq) > 1e-30 ? (float)(st->rs_old / dq) : 0.0f;
@@ -561,7 +330,7 @@ __global__ __launch_bounds__(kCgMultiThreads) void k_cg_phase_b(StepParams P, co
     auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
-        acc += phase_b_batch(P, dn, q, p, r, alpha, tw, ti, tile_id);
+        acc += phase_b_batch<kU>(P, dn, q, p, r, alpha, tw, ti, tile_id);
     double rr;
     if (last_block_sum<kCgMultiThreads>(acc, w, rr) && threadIdx.x == 0) {
         const double rs_old = st->rs_old;
@@ -580,7 +349,7 @@ __global__ __launch_bounds__(kCgMultiThreads) void k_cg_true_res(StepParams P, c
     auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
-        acc += true_res_batch(P, w.nmask, b, p, tw, ti, tile_id);
+        acc += true_res_batch<kU>(P, w.nmask, b, p, tw, ti, tile_id);
     double tr;
     if (last_block_sum<kCgMultiThreads>(acc, w, tr) && threadIdx.x == 0) {
         w.state->true_res_sq = tr;

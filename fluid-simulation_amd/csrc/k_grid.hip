@@ -9,8 +9,8 @@
 //     kernel, and gradient + enforce as one pass over all faces.
 // Both forms evaluate the per-face functions of fs_face.h, so they agree bit
 // for bit.
-#include "fs_kernels.h"
-#include "fs_face.h"
+#include "fs_cell.h"
+#include "fs_gather.h"
 
 namespace fs {
 
@@ -24,20 +24,9 @@ __global__ void k_classify(StepParams P, const uint32_t* __restrict__ occ, uint8
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     bool liquid = false;
     if (c < P.C) {
-        const uint32_t nx = P.nx, ny = P.ny, nz = P.nz;
-        const uint32_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-        const bool boundary = i == 0u || j == 0u || k == 0u || i == nx - 1u || j == ny - 1u || k == nz - 1u;
-        if (boundary) {
-            ct[c] = kSolid;
-        } else {
-            const uint32_t t = P.thresh;
-            liquid = occ[c] >= t;
-            if (!liquid && P.dilation) {
-                liquid = occ[c - 1u] >= t || occ[c + 1u] >= t || occ[c - nx] >= t || occ[c + nx] >= t ||
-                         occ[c - nx * ny] >= t || occ[c + nx * ny] >= t;
-            }
-            ct[c] = liquid ? kLiquid : kAir;
-        }
+        const uint8_t t = classify_cell(P, occ, c);
+        ct[c] = t;
+        liquid = t == kLiquid;
     }
     // One atomic per block (the count is an integer sum, so grouping is free).
     __shared__ uint32_t s_cnt;
@@ -59,46 +48,29 @@ hipError_t launch_classify(const StepParams& P, const uint32_t* occ, uint8_t* ct
 template <int BF16>
 __global__ void k_p2g_atomic(StepParams P, ParticleSoA sp, int32_t* __restrict__ un, int32_t* __restrict__ ud,
                              int32_t* __restrict__ vn, int32_t* __restrict__ vd, int32_t* __restrict__ wn,
-                             int32_t* __restrict__ wd, const uint32_t* __restrict__ guard) {
-    if (guard && *guard == 0u) return;
-    int32_t* nums[3] = {un, vn, wn};
-    int32_t* dens[3] = {ud, vd, wd};
+                             int32_t* __restrict__ wd, const uint32_t* __restrict__ guard, uint32_t above) {
+    if (guard && *guard <= above) return;
+    int32_t* const nums[3] = {un, vn, wn};
+    int32_t* const dens[3] = {ud, vd, wd};
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P.N; p += stride) {
-        const float pos[3] = {(sp.pos[0][p] - P.origin[0]) * P.inv_h, (sp.pos[1][p] - P.origin[1]) * P.inv_h,
-                              (sp.pos[2][p] - P.origin[2]) * P.inv_h};
-        const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
-                             load_vel<BF16>(sp.vel[2], p)};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            uint32_t dim[3];
-            face_dims(P, a, dim);
-            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
-            int32_t* nm = nums[a];
-            int32_t* dn = dens[a];
-            p2g_taps(g, dim, pv[a], P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
-                const uint32_t idx = (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk);
-                atomicAdd(&nm[idx], tn);
-                atomicAdd(&dn[idx], td);
-            });
-        }
-    }
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P.N; p += stride)
+        p2g_atomic_particle<BF16>(P, sp, p, nums, dens);
 }
 
 // A guarded launch is the fused engine's fallback and is usually a no-op, so
 // it uses a small grid-stride grid; unguarded launches take one particle per
 // thread like sh/scatter.wgsl.
 hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf16, int32_t* const num[3],
-                             int32_t* const den[3], const uint32_t* guard, hipStream_t s) {
+                             int32_t* const den[3], const uint32_t* guard, uint32_t above, hipStream_t s) {
     if (P.N == 0) return hipSuccess;
     unsigned g = grid_for(P.N, 64);
     if (guard && g > 4096) g = 4096;
     if (bf16)
         hipLaunchKernelGGL(k_p2g_atomic<1>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2],
-                           den[2], guard);
+                           den[2], guard, above);
     else
         hipLaunchKernelGGL(k_p2g_atomic<0>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2],
-                           den[2], guard);
+                           den[2], guard, above);
     return hipGetLastError();
 }
 
@@ -168,33 +140,12 @@ hipError_t launch_gradient(const StepParams& P, int a, const float* p, const uin
     return hipGetLastError();
 }
 
-// sh/divergence.wgsl:22-59.
-__device__ __forceinline__ float cell_divergence(const StepParams& P, const float* __restrict__ u,
-                                                 const float* __restrict__ v, const float* __restrict__ w,
-                                                 const uint32_t* __restrict__ occ, uint32_t c) {
-    const uint32_t nx = P.nx, ny = P.ny;
-    const uint32_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-    const float u_hi = u[(i + 1u) + (nx + 1u) * (j + ny * k)];
-    const float u_lo = u[i + (nx + 1u) * (j + ny * k)];
-    const float v_hi = v[i + nx * ((j + 1u) + (ny + 1u) * k)];
-    const float v_lo = v[i + nx * (j + (ny + 1u) * k)];
-    const float w_hi = w[i + nx * (j + ny * (k + 1u))];
-    const float w_lo = w[i + nx * (j + ny * k)];
-    float div = ((u_hi - u_lo) + (v_hi - v_lo) + (w_hi - w_lo)) * P.inv_h;
-    if (P.stiff > 0.0f && P.rest > 0.0f) {
-        const float over = fmaxf(0.0f, (float)occ[c] - P.rest) / P.rest;
-        const float drift = fminf(P.stiff * over, P.drift);
-        div = div - drift;
-    }
-    return div;
-}
-
 __global__ void k_divergence(StepParams P, const float* __restrict__ u, const float* __restrict__ v,
                              const float* __restrict__ w, const uint8_t* __restrict__ ct,
                              const uint32_t* __restrict__ occ, float* __restrict__ div) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.C) return;
-    div[c] = ct[c] != kLiquid ? 0.0f : cell_divergence(P, u, v, w, occ, c);
+    div[c] = ct[c] != kLiquid ? 0.0f : cell_div(P, u, v, w, occ, c);
 }
 
 hipError_t launch_divergence(const StepParams& P, float* const vel[3], const uint8_t* ct, const uint32_t* occ,
@@ -205,22 +156,13 @@ hipError_t launch_divergence(const StepParams& P, float* const vel[3], const uin
 }
 
 // ---- fused face passes over the concatenated u|v|w face index space ----
-__device__ __forceinline__ bool split_face(const StepParams& P, uint32_t g, int& a, uint32_t& f) {
-    if (g < P.F[0]) { a = 0; f = g; return true; }
-    g -= P.F[0];
-    if (g < P.F[1]) { a = 1; f = g; return true; }
-    g -= P.F[1];
-    if (g < P.F[2]) { a = 2; f = g; return true; }
-    return false;
-}
-
 __global__ void k_face_epilogue(StepParams P, const int32_t* __restrict__ n0, const int32_t* __restrict__ n1,
                                 const int32_t* __restrict__ n2, const int32_t* __restrict__ d0,
                                 const int32_t* __restrict__ d1, const int32_t* __restrict__ d2,
                                 const uint8_t* __restrict__ ct, float* __restrict__ v0, float* __restrict__ v1,
                                 float* __restrict__ v2, float* __restrict__ s0, float* __restrict__ s1,
-                                float* __restrict__ s2, const uint32_t* __restrict__ guard) {
-    if (guard && *guard == 0u) return;
+                                float* __restrict__ s2, const uint32_t* __restrict__ guard, uint32_t above) {
+    if (guard && *guard <= above) return;
     const uint32_t tot = P.F[0] + P.F[1] + P.F[2];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < tot; gi += stride) {
@@ -243,12 +185,12 @@ __global__ void k_face_epilogue(StepParams P, const int32_t* __restrict__ n0, co
 
 hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3], const int32_t* const den[3],
                                 const uint8_t* ct, float* const vel[3], float* const saved[3], const uint32_t* guard,
-                                hipStream_t s) {
+                                uint32_t above, hipStream_t s) {
     const size_t tot = (size_t)P.F[0] + P.F[1] + P.F[2];
     unsigned g = grid_for(tot, 256);
     if (guard && g > 4096) g = 4096;  // usually a no-op fallback: small grid-stride grid
     hipLaunchKernelGGL(k_face_epilogue, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
-                       ct, vel[0], vel[1], vel[2], saved[0], saved[1], saved[2], guard);
+                       ct, vel[0], vel[1], vel[2], saved[0], saved[1], saved[2], guard, above);
     return hipGetLastError();
 }
 
@@ -275,163 +217,55 @@ hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const ui
 }
 
 // ---------------------------------------------------------------------------
-// Sorted-segment P2G gather.  One workgroup owns a brick of BX x BY x BZ cells
-// and every MAC face whose cell coordinate (own-axis coordinate clamped to
-// n-1) lies in the brick.  A face's taps come from particles whose cell is in
-// the brick grown by one cell on every side; after the sort those particles
-// are, per (y,z) row of that halo region, one contiguous segment of the
-// sorted arrays.  The workgroup walks the segments, evaluates each particle's
-// 24 taps exactly as sh/scatter.wgsl does, and adds the ones it owns into LDS
-// with integer atomics (order-free, so the sums equal the reference's i32
-// atomics bit for bit).  The epilogue normalizes, saves, applies gravity and
-// enforces walls per owned face, and stores vel/saved once.
-//
-// Requires every particle's P2G cell floor((pos-origin)*inv_h) to equal its
-// sort cell (clamped floor((pos-origin)/h)); the host checks the count of
-// particles breaking that (k_misfit_count) and uses the atomic form otherwise.
-namespace {
-constexpr int kGBX = 16, kGBY = 8, kGBZ = 8;
-constexpr int kGThreads = 256;
-constexpr int kGRowsMax = (kGBY + 2) * (kGBZ + 2);
-constexpr int kGU = (kGBX + 1) * kGBY * kGBZ;
-constexpr int kGV = kGBX * (kGBY + 1) * kGBZ;
-constexpr int kGW = kGBX * kGBY * (kGBZ + 1);
-}  // namespace
-
-template <int BF16>
-__global__ __launch_bounds__(kGThreads) void k_p2g_gather(StepParams P, ParticleSoA sp,
+// Sorted-segment P2G gather (fs_gather.h), one workgroup per brick.  When more
+// particles were listed as misfits than the gather walks, it exits and the
+// atomic form runs instead.
+template <int BF16, int V>
+__global__ __launch_bounds__(kGThreads) void k_p2g_gather(StepParams P, ParticleSoA sp, ParticleSoA src,
                                                            const uint32_t* __restrict__ occ,
                                                            const uint32_t* __restrict__ off,
-                                                           const uint8_t* __restrict__ ct, float* __restrict__ v0,
-                                                           float* __restrict__ v1, float* __restrict__ v2,
-                                                           float* __restrict__ s0, float* __restrict__ s1,
-                                                           float* __restrict__ s2, uint32_t nbx, uint32_t nby,
-                                                           uint32_t nbz, const uint32_t* __restrict__ misfit) {
-    if (*misfit != 0u) return;  // the atomic form handles this step
-    __shared__ int32_t s_num[kGU + kGV + kGW];
-    __shared__ int32_t s_den[kGU + kGV + kGW];
-    __shared__ uint32_t s_row_start[kGRowsMax];
-    __shared__ uint32_t s_row_pref[kGRowsMax + 1];
-
+                                                           const uint8_t* __restrict__ ct, GatherOut out,
+                                                           uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                                           const uint32_t* __restrict__ mis_count,
+                                                           const uint32_t* __restrict__ mis_list) {
+    const uint32_t mis_n = *mis_count;
+    if (mis_n > kMisfitCap) return;
     const uint32_t G = nbx * nby * nbz;
     const uint32_t b = xcd_remap(blockIdx.x, G);
     const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
-    const int32_t x0 = (int32_t)bxi * kGBX, y0 = (int32_t)byi * kGBY, z0 = (int32_t)bzi * kGBZ;
-    const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
-
-    for (int t = threadIdx.x; t < kGU + kGV + kGW; t += kGThreads) {
-        s_num[t] = 0;
-        s_den[t] = 0;
-    }
-    // Halo cell range (inclusive), clamped to the grid.
-    const int32_t xs = max(x0 - 1, 0), xe = min(x0 + kGBX, nx - 1);
-    const int32_t ys = max(y0 - 1, 0), ye = min(y0 + kGBY, ny - 1);
-    const int32_t zs = max(z0 - 1, 0), ze = min(z0 + kGBZ, nz - 1);
-    const int32_t rows_y = ye - ys + 1;
-    const int32_t rows = rows_y * (ze - zs + 1);
-    if (threadIdx.x < 64) {
-        // Row segments and their prefix sums (one wave, rows <= 100).
-        uint32_t carry = 0;
-        for (int r0 = 0; r0 < rows; r0 += 64) {
-            const int r = r0 + (int)threadIdx.x;
-            uint32_t len = 0, start = 0;
-            if (r < rows) {
-                const int32_t cy = ys + r % rows_y, cz = zs + r / rows_y;
-                const uint32_t cf = (uint32_t)xs + P.nx * ((uint32_t)cy + P.ny * (uint32_t)cz);
-                const uint32_t cl = (uint32_t)xe + P.nx * ((uint32_t)cy + P.ny * (uint32_t)cz);
-                start = off[cf];
-                len = off[cl] + occ[cl] - start;
-                s_row_start[r] = start;
-            }
-            uint32_t inc = len;
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t t = __shfl_up(inc, o, 64);
-                if ((int)threadIdx.x >= o) inc += t;
-            }
-            if (r < rows) s_row_pref[r + 1] = carry + inc;
-            carry += __shfl(inc, 63, 64);
-        }
-        if (threadIdx.x == 0) s_row_pref[0] = 0;
-    }
-    __syncthreads();
-    const uint32_t total = s_row_pref[rows];
-    const int32_t ownx_hi = (x0 + kGBX >= nx) ? kGBX : kGBX - 1;  // own-axis local max index per axis
-    const int32_t owny_hi = (y0 + kGBY >= ny) ? kGBY : kGBY - 1;
-    const int32_t ownz_hi = (z0 + kGBZ >= nz) ? kGBZ : kGBZ - 1;
-    int row = 0;
-    for (uint32_t base = 0; base < total; base += kGThreads) {
-        // Each wave takes every 4th item of the chunk, so its lanes fall in
-        // different cells (sorted particles of one cell share faces, and
-        // same-address LDS atomics within a wave serialize).
-        const uint32_t t = base + (threadIdx.x & 63u) * (kGThreads / 64) + (threadIdx.x >> 6);
-        if (t >= total) continue;
-        // Find the row holding flattened item t (rows are few; linear advance).
-        while (s_row_pref[row + 1] <= t) ++row;
-        const uint32_t p = s_row_start[row] + (t - s_row_pref[row]);
-        const float pos[3] = {(sp.pos[0][p] - P.origin[0]) * P.inv_h, (sp.pos[1][p] - P.origin[1]) * P.inv_h,
-                              (sp.pos[2][p] - P.origin[2]) * P.inv_h};
-        const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
-                             load_vel<BF16>(sp.vel[2], p)};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            uint32_t dim[3];
-            face_dims(P, a, dim);
-            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
-            const int32_t hx = a == 0 ? ownx_hi : kGBX - 1;
-            const int32_t hy = a == 1 ? owny_hi : kGBY - 1;
-            const int32_t hz = a == 2 ? ownz_hi : kGBZ - 1;
-            const int32_t lx_dim = a == 0 ? kGBX + 1 : kGBX;
-            const int32_t ly_dim = a == 1 ? kGBY + 1 : kGBY;
-            const int32_t base = a == 0 ? 0 : (a == 1 ? kGU : kGU + kGV);
-            p2g_taps(g, dim, pv[a], P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
-                const int32_t lx = ii - x0, ly = jj - y0, lz = kk - z0;
-                if (lx < 0 || lx > hx || ly < 0 || ly > hy || lz < 0 || lz > hz) return;
-                const int32_t li = base + lx + lx_dim * (ly + ly_dim * lz);
-                atomicAdd(&s_num[li], tn);
-                atomicAdd(&s_den[li], td);
-            });
-        }
-    }
-    __syncthreads();
-    // Epilogue over owned faces of the three axes.
-    for (int a = 0; a < 3; ++a) {
-        const int32_t lxd = a == 0 ? kGBX + 1 : kGBX;
-        const int32_t lyd = a == 1 ? kGBY + 1 : kGBY;
-        const int32_t lzd = a == 2 ? kGBZ + 1 : kGBZ;
-        const int32_t hx = a == 0 ? ownx_hi : kGBX - 1;
-        const int32_t hy = a == 1 ? owny_hi : kGBY - 1;
-        const int32_t hz = a == 2 ? ownz_hi : kGBZ - 1;
-        const int32_t base = a == 0 ? 0 : (a == 1 ? kGU : kGU + kGV);
-        float* vel = a == 0 ? v0 : (a == 1 ? v1 : v2);
-        float* sav = a == 0 ? s0 : (a == 1 ? s1 : s2);
-        uint32_t dim[3];
-        face_dims(P, a, dim);
-        for (int t = threadIdx.x; t < lxd * lyd * lzd; t += kGThreads) {
-            const int32_t lx = t % lxd, ly = (t / lxd) % lyd, lz = t / (lxd * lyd);
-            if (lx > hx || ly > hy || lz > hz) continue;
-            const int32_t i = x0 + lx, j = y0 + ly, k = z0 + lz;
-            if (i >= (int32_t)dim[0] || j >= (int32_t)dim[1] || k >= (int32_t)dim[2]) continue;
-            float vv, ss;
-            face_epilogue(P, a, ct, (uint32_t)i, (uint32_t)j, (uint32_t)k, s_num[base + t], s_den[base + t], vv, ss);
-            const uint32_t f = (uint32_t)i + dim[0] * ((uint32_t)j + dim[1] * (uint32_t)k);
-            vel[f] = vv;
-            sav[f] = ss;
-        }
-    }
+    p2g_gather_brick<BF16, V>(P, sp, src, occ, off, ct, out, (int32_t)bxi * Brick<V>::X, (int32_t)byi * Brick<V>::Y,
+                              (int32_t)bzi * Brick<V>::Z, mis_list, mis_n, 0u, P.N);
 }
 
-hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, int bf16, const uint32_t* occ,
-                             const uint32_t* cell_offset, const uint8_t* ct, float* const vel[3],
-                             float* const saved[3], const uint32_t* misfit, hipStream_t s) {
-    const uint32_t nbx = (P.nx + kGBX - 1) / kGBX, nby = (P.ny + kGBY - 1) / kGBY, nbz = (P.nz + kGBZ - 1) / kGBZ;
+namespace {
+template <int V>
+hipError_t launch_gather_v(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
+                           const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct, const GatherOut& out,
+                           const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s) {
+    const uint32_t nbx = (P.nx + Brick<V>::X - 1) / Brick<V>::X, nby = (P.ny + Brick<V>::Y - 1) / Brick<V>::Y,
+                   nbz = (P.nz + Brick<V>::Z - 1) / Brick<V>::Z;
     const uint32_t G = nbx * nby * nbz;
     if (bf16)
-        hipLaunchKernelGGL(k_p2g_gather<1>, dim3(G), dim3(kGThreads), 0, s, P, sorted, occ, cell_offset, ct, vel[0],
-                           vel[1], vel[2], saved[0], saved[1], saved[2], nbx, nby, nbz, misfit);
+        hipLaunchKernelGGL((k_p2g_gather<1, V>), dim3(G), dim3(kGThreads), 0, s, P, sorted, src, occ, cell_offset, ct,
+                           out, nbx, nby, nbz, mis_count, mis_list);
     else
-        hipLaunchKernelGGL(k_p2g_gather<0>, dim3(G), dim3(kGThreads), 0, s, P, sorted, occ, cell_offset, ct, vel[0],
-                           vel[1], vel[2], saved[0], saved[1], saved[2], nbx, nby, nbz, misfit);
+        hipLaunchKernelGGL((k_p2g_gather<0, V>), dim3(G), dim3(kGThreads), 0, s, P, sorted, src, occ, cell_offset, ct,
+                           out, nbx, nby, nbz, mis_count, mis_list);
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
+                             const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct, float* const vel[3],
+                             float* const saved[3], const uint32_t* mis_count, const uint32_t* mis_list,
+                             hipStream_t s, int brick_variant) {
+    const GatherOut out = {{vel[0], vel[1], vel[2]}, {saved[0], saved[1], saved[2]}};
+    switch (brick_variant) {
+        case 0: return launch_gather_v<0>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, s);
+        case 1: return launch_gather_v<1>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, s);
+        case 2: return launch_gather_v<2>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace fs

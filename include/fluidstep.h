@@ -182,6 +182,55 @@ FS_API uint64_t fs_buffer_bytes(const fs_tank* t);
 FS_API const char* fs_last_error(void);
 FS_API const char* fs_version(void);
 
+/* ---- batched tanks ----
+ * Many independent tanks of uneven sizes stepped together on one GPU.  Each
+ * kernel phase of a substep is one launch over every tank (a tank table and
+ * per-phase work tables on the device); the pressure solve is one persistent
+ * launch in which each tank's blocks run that tank's CG with their own
+ * reductions; the whole substep is replayed from a captured hipGraph.  Per
+ * tank the result is the fused engine's step (FS_ENGINE_FUSED,
+ * FS_PRECISION_MIXED, particles sorted every substep); velocity storage must
+ * be the same for every tank.  The reference steps one GpuFluid per
+ * GpuContext (src/gpu/mod.rs:950-1033); a batch replaces a loop of
+ * GpuFluid::new / GpuContext::step over tanks. */
+typedef struct fs_batch fs_batch;
+
+/* Tile shapes the autotuner picks from (each an index into a fixed menu). */
+typedef struct fs_batch_tuning {
+    uint32_t gather_brick; /* P2G gather brick: 0 = 16x8x8, 1 = 8x8x8, 2 = 32x4x4 cells */
+    uint32_t sort_tile;    /* radix-sort keys per workgroup: 0 = 4096, 1 = 2048, 2 = 8192 */
+    uint32_t cg_unroll;    /* CG tiles in flight per wave: 0 = 4, 1 = 2, 2 = 8 */
+    uint32_t cg_blocks;    /* persistent CG workgroups (0 = default) */
+    double ms_gather;      /* best measured time of each tuned phase (ms) */
+    double ms_sort;
+    double ms_cg;
+} fs_batch_tuning;
+
+/* params[i], pos_xyz[i], vel_xyz[i] (NULL = zero), counts[i] describe tank i. */
+FS_API int fs_batch_create(const fs_params* params, const float* const* pos_xyz, const float* const* vel_xyz,
+                           const uint32_t* counts, uint32_t ntanks, fs_batch** out);
+FS_API int fs_batch_destroy(fs_batch* b);
+/* n substeps of every tank (replays the captured substep graph). */
+FS_API int fs_batch_step(fs_batch* b, uint32_t substeps);
+FS_API int fs_batch_synchronize(fs_batch* b);
+FS_API int fs_batch_set_timing(fs_batch* b, int enabled);
+/* Time each menu entry of the tuned phases on the batch's current state
+ * (restored afterwards) and keep the fastest; out (nullable) receives the
+ * choice.  Without a call the first menu entries are used. */
+FS_API int fs_batch_autotune(fs_batch* b, fs_batch_tuning* out);
+FS_API int fs_batch_set_tuning(fs_batch* b, const fs_batch_tuning* tuning);
+FS_API uint32_t fs_batch_tank_count(const fs_batch* b);
+FS_API uint64_t fs_batch_buffer_bytes(const fs_batch* b);
+/* Read-outs of tank `tank`, laid out as the single-tank read-outs; the
+ * permutation holds tank-local indices.  Stats: per-tank solver figures, the
+ * batch's phase times. */
+FS_API int fs_batch_read_particles(fs_batch* b, uint32_t tank, float* pos_xyz, float* vel_xyz);
+FS_API int fs_batch_read_faces(fs_batch* b, uint32_t tank, int axis, int which, float* out);
+FS_API int fs_batch_read_cells_f32(fs_batch* b, uint32_t tank, int which, float* out);
+FS_API int fs_batch_read_cells_u32(fs_batch* b, uint32_t tank, int which, uint32_t* out);
+FS_API int fs_batch_read_permutation(fs_batch* b, uint32_t tank, uint32_t* perm);
+FS_API int fs_batch_get_stats(fs_batch* b, uint32_t tank, fs_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,21 +125,34 @@ def _misfit_coordinate(n):
     return None
 
 
+def _brick_edge_misfit_24():
+    """At n = 24, y = f32(0.29166666) sorts into cell 6 (y/h) but scatters
+    from cell 7 (y*inv_h): its v taps reach face j = 8, owned by the gather
+    brick starting at y = 8, whose halo (cells 7..16) does not hold cell 6."""
+    h = np.float32(1.0) / np.float32(24)
+    y = np.float32(0.29166666)
+    assert np.floor(y / h) == 6 and np.floor(y * (np.float32(1.0) / h)) == 7
+    return float(y)
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 def test_non_power_of_two_grid_with_misfit(engine):
-    """24^3 (h = 1/24): a particle where the sort cell (divide by h) and the
-    P2G cell (multiply by 1/h) differ forces the fused engine onto its atomic
-    P2G fallback; results still match."""
+    """24^3 (h = 1/24): particles whose sort cell (divide by h) and P2G cell
+    (multiply by 1/h) differ, one of them reaching a gather brick from outside
+    its halo.  The fused engine's gather adds them from the misfit list;
+    results still match bit for bit on the P2G side."""
     sc = scenes.small_tank(n=24, seed=11, fill=0.45)
     x = _misfit_coordinate(24)
     assert x is not None
     pos = sc["pos"].copy()
     pos[5, 0] = x
+    pos[6, 1] = _brick_edge_misfit_24()
+    pos[7, 2] = _brick_edge_misfit_24()
     say("misfit", sc["name"], "engine", engine, "x", x)
     ref, t = run_pair(sc, engine, pos=pos)
     st = t.stats()
     if engine == fsm.ENGINE_FUSED:
-        assert st["misfits"] >= 1 and st["p2g_gather"] == 0
+        assert st["misfits"] >= 3 and st["p2g_gather"] == 1
     assert np.array_equal(t.permutation(), ref.perm)
     saved = t.faces(fsm.FACE_SAVED)
     for a in range(3):
@@ -147,6 +160,25 @@ def test_non_power_of_two_grid_with_misfit(engine):
     assert rel_l2(t.pressure(), ref.pressure) <= TOL
     gp, gv = t.particles()
     assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+    t.close()
+
+
+def test_misfit_list_overflow_takes_atomic_form():
+    """More misfit particles than the gather's list holds (4096): the fused
+    engine switches to the atomic P2G form for the step; results match."""
+    sc = scenes.small_tank(n=24, seed=12, fill=0.45)
+    pos = sc["pos"].copy()
+    assert len(pos) > 6000
+    pos[:5000, 1] = _brick_edge_misfit_24()
+    say("misfit overflow", sc["name"], "N", len(pos))
+    ref, t = run_pair(sc, fsm.ENGINE_FUSED, pos=pos)
+    st = t.stats()
+    assert st["misfits"] >= 5000 and st["p2g_gather"] == 0
+    assert np.array_equal(t.permutation(), ref.perm)
+    saved = t.faces(fsm.FACE_SAVED)
+    for a in range(3):
+        assert bits_equal(saved[a], ref.saved[a])
+    assert rel_l2(t.pressure(), ref.pressure) <= TOL
     t.close()
 
 
