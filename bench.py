@@ -82,6 +82,86 @@ def cpu_baseline(sc, steps):
     return dt, res, oracle.num_threads()
 
 
+def _pct(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else None
+
+
+def batch_bench(fsm, scenes, gpu_params, args):
+    """The 64-tank batch: aggregate particle-steps/s and per-tank step
+    latency, against the same 64 tanks stepped one after another as single
+    tanks (same fused engine), and against the plain FP32 single-tank step."""
+    tanks = scenes.batched_tanks()
+    counts = [len(sc["pos"]) for sc in tanks]
+    N = int(sum(counts))
+    params = [gpu_params(sc, fsm.ENGINE_FUSED) for sc in tanks]
+    b = fsm.FluidBatch(params, [sc["pos"] for sc in tanks], [sc["vel"] for sc in tanks])
+    b.step(max(args.warmup, 1))
+    b.synchronize()
+    tuning = None
+    if not args.no_tune:
+        tuning = b.autotune()
+        print("batch tuning", json.dumps(tuning), file=sys.stderr, flush=True)
+        b.step(1)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    value = N * args.steps / dt
+    # Per-step device time (every tank's step ends with the batch's step).
+    b.set_timing(True)
+    lat = []
+    for _ in range(args.latency_steps):
+        b.step(1)
+        lat.append(b.stats(0)["ms_total"])
+    iters = [b.stats(t)["cg_iterations"] for t in range(len(tanks))]
+    b.close()
+
+    # The same tanks one after another, each its own single-tank launch sequence.
+    def sequential(engine, steps):
+        hs = [fsm.FluidStep(gpu_params(sc, engine), sc["pos"], sc["vel"]) for sc in tanks]
+        for h in hs:
+            h.step(1)
+            h.synchronize()
+        per_tank = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for h in hs:
+                s0 = time.perf_counter()
+                h.step(1)
+                h.synchronize()
+                per_tank.append((time.perf_counter() - s0) * 1e3)
+        sdt = time.perf_counter() - t0
+        for h in hs:
+            h.close()
+        return N * steps / sdt, per_tank
+
+    seq_v, seq_lat = sequential(fsm.ENGINE_FUSED, args.seq_steps)
+    line = {
+        "metric": "batched tanks: aggregate FLIP particle-steps/s (64 tanks, one launch per phase, hipGraph)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "precision": "mixed: f32 vectors and particles, f64 CG dots/scalars/true residual",
+        "data": "synthetic",
+        "config": {"workload": "batched_tanks", "tanks": len(tanks), "particles": N,
+                   "edges": [sc["n"] for sc in tanks], "dt": tanks[0]["physics"]["dt"], "cg_tolerance": 1e-5,
+                   "cg_cap": 200, "sort_period": 1, "engine": "fused batch"},
+        "tuning": tuning,
+        "latency_ms": {"batch_step_p50": _pct(lat, 50), "batch_step_p99": _pct(lat, 99),
+                       "note": "every tank of the batch completes its step when the batch step completes"},
+        "cg_iterations": {"min": int(min(iters)), "max": int(max(iters)), "mean": float(np.mean(iters))},
+        "sequential_fused": {"value": seq_v, "unit": "particle-steps/s",
+                             "tank_step_ms_p50": _pct(seq_lat, 50), "tank_step_ms_p99": _pct(seq_lat, 99)},
+        "speedup_vs_sequential_fused": value / seq_v,
+    }
+    if not args.no_plain:
+        pv, plat = sequential(fsm.ENGINE_REFERENCE, 1)
+        line["sequential_plain_fp32"] = {"value": pv, "unit": "particle-steps/s",
+                                         "tank_step_ms_p50": _pct(plat, 50), "tank_step_ms_p99": _pct(plat, 99)}
+        line["speedup_vs_sequential_plain_fp32"] = value / pv
+    return line
+
+
 def _track(ids, perm):
     """Compose the sort permutation into the particle-identity array."""
     return ids[perm] if perm is not None else ids
@@ -177,8 +257,12 @@ def main():
                          "restatement after STEPS steps (and one-step errors from its states)")
     ap.add_argument("--teacher-states", type=int, default=3)
     ap.add_argument("--workload", default="high_density_128",
-                    choices=["high_density_128", "large_splash_256", "dam_break_32", "control_64"],
-                    help="the metric is quoted on high_density_128; the others are for per-kernel profiles")
+                    choices=["high_density_128", "large_splash_256", "dam_break_32", "control_64", "batched_tanks"],
+                    help="the metric is quoted on high_density_128; the others are for per-kernel profiles "
+                         "and the batched-tank figures")
+    ap.add_argument("--no-tune", action="store_true", help="batched_tanks: skip the autotuner")
+    ap.add_argument("--latency-steps", type=int, default=20, help="batched_tanks: steps timed one by one")
+    ap.add_argument("--seq-steps", type=int, default=3, help="batched_tanks: steps of the sequential baseline")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -194,6 +278,10 @@ def main():
 
     if args.precision_report:
         print(json.dumps(precision_report(args.precision_report, args.teacher_states)), flush=True)
+        return
+
+    if args.workload == "batched_tanks":
+        print(json.dumps(batch_bench(fsm, scenes, gpu_params, args)), flush=True)
         return
 
     sc = getattr(scenes, args.workload)()

@@ -43,9 +43,10 @@ struct BatchCgWork {
     uint32_t* tile_scan;   // [tiles_tot + 1]
     uint32_t* tiles;       // [tiles_tot] active tile list (global tile ids)
     uint32_t* n_tiles;     // [1]
-    double* partials;      // [2 * grid]
-    uint32_t* arrive;      // [T * 32]: one arrival counter per 128-byte line
-    uint32_t* gen;         // [T * 32]: one generation word per 128-byte line
+    double* partials;      // [2 * grid] one slot per workgroup, two halves
+    uint32_t* flags;       // [2 * grid] the slot's tank still iterates
+    uint32_t* arrive;      // [9 * 32]: eight group counters and the grid counter, one 128-byte line each
+    uint32_t* gen;         // [32]: grid barrier generation
     uint32_t* abort_flag;  // [1]
     BatchCgState* state;   // [T]
 };
@@ -77,7 +78,7 @@ hipError_t launch_batch_div_cginit(const BatchTank* T, const WorkItem* wl, uint3
                                    const uint8_t* ct, const uint32_t* occ, float* div, float* b, float* p, float* r,
                                    float* d, const BatchCgWork& w, hipStream_t s);
 // One cooperative launch; each tank's group of workgroups runs that tank's
-// solve.  cg_unroll: 0 = 4, 1 = 2, 2 = 8 tiles in flight per wave.
+// solve.  cg_unroll: 0 = 4, 1 = 2, 2 = 1 tiles in flight per wave.
 hipError_t launch_batch_cg(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int cg_unroll, const float* b,
                            float* p, float* r, float* d0, float* d1, float* q, const BatchCgWork& w, hipStream_t s);
 int batch_cg_max_blocks();

@@ -76,6 +76,9 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
         c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
         m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
     }
+    // Every value load is issued without waiting for the mask (a neighbour
+    // index is clamped into the grid); the mask selects the terms below, so a
+    // wave pays one memory round trip per batch of tiles instead of two.
     float rc[U], dc[U], rn[U][6], dnb[U][6];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -84,10 +87,10 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
         dc[u] = ok ? dprev[c[u]] : 0.0f;
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const bool lq = (m[u] >> (3 + s)) & 1u;
-            const uint32_t nbc = (uint32_t)((int32_t)c[u] + off[s]);
-            rn[u][s] = (lq && !first) ? r[nbc] : 0.0f;
-            dnb[u][s] = lq ? dprev[nbc] : 0.0f;
+            int32_t nb = (int32_t)c[u] + off[s];
+            nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
+            rn[u][s] = (ok && !first) ? r[nb] : 0.0f;
+            dnb[u][s] = ok ? dprev[nb] : 0.0f;
         }
     }
     double acc = 0.0;

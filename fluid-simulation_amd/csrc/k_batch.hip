@@ -8,9 +8,9 @@
 //
 // The pressure solve is one cooperative launch.  Its workgroups are split into
 // one group per tank (sized on the host by the tank's expected liquid volume);
-// a group runs that tank's mixed-precision CG with its own arrival counter,
-// generation word and partial slots, so tanks never wait for each other and a
-// tank that converges early releases its workgroups.
+// a group runs that tank's mixed-precision CG over the tank's liquid tiles
+// and reduces over its own partial slots, while the grid barriers are shared
+// by all tanks (lock step); a tank that has converged idles at the barriers.
 #include "fs_batch.h"
 #include "fs_cell.h"
 #include "fs_gather.h"
@@ -145,43 +145,52 @@ hipError_t launch_batch_gather(const BatchTank* T, const WorkItem* wl, uint32_t 
 }
 
 // ---------------------------------------------------------------------------
-// Fallback: atomic P2G and the face epilogue, when too many misfits.
+// Fallback: atomic P2G and the face epilogue, when too many misfits.  Both
+// are usually no-ops, so they walk their work tables with a small
+// grid-stride grid.
+constexpr uint32_t kFallbackGrid = 2048;
+
 template <int BF16>
 __global__ __launch_bounds__(kBatchBlock) void kb_p2g_atomic(const BatchTank* __restrict__ T,
-                                                             const WorkItem* __restrict__ wl, ParticleSoA sp,
-                                                             BatchGrid g, const uint32_t* __restrict__ mis_count) {
+                                                             const WorkItem* __restrict__ wl, uint32_t nwl,
+                                                             ParticleSoA sp, BatchGrid g,
+                                                             const uint32_t* __restrict__ mis_count) {
     if (*mis_count <= kMisfitCap) return;
-    const WorkItem wi = wl[blockIdx.x];
-    if (wi.tank == kNoTank) return;
-    const BatchTank& bt = T[wi.tank];
-    const StepParams P = bt.P;
-    const uint32_t local = wi.start + threadIdx.x;
-    if (local >= P.N) return;
-    int32_t* const nums[3] = {g.num[0] + bt.face_off[0], g.num[1] + bt.face_off[1], g.num[2] + bt.face_off[2]};
-    int32_t* const dens[3] = {g.den[0] + bt.face_off[0], g.den[1] + bt.face_off[1], g.den[2] + bt.face_off[2]};
-    p2g_atomic_particle<BF16>(P, sp, bt.part_off + local, nums, dens);
+    for (uint32_t w = blockIdx.x; w < nwl; w += gridDim.x) {
+        const WorkItem wi = wl[w];
+        if (wi.tank == kNoTank) continue;
+        const BatchTank& bt = T[wi.tank];
+        const StepParams P = bt.P;
+        const uint32_t local = wi.start + threadIdx.x;
+        if (local >= P.N) continue;
+        int32_t* const nums[3] = {g.num[0] + bt.face_off[0], g.num[1] + bt.face_off[1], g.num[2] + bt.face_off[2]};
+        int32_t* const dens[3] = {g.den[0] + bt.face_off[0], g.den[1] + bt.face_off[1], g.den[2] + bt.face_off[2]};
+        p2g_atomic_particle<BF16>(P, sp, bt.part_off + local, nums, dens);
+    }
 }
 
 __global__ __launch_bounds__(kBatchBlock) void kb_face_epilogue(const BatchTank* __restrict__ T,
-                                                                const WorkItem* __restrict__ wl,
+                                                                const WorkItem* __restrict__ wl, uint32_t nwl,
                                                                 const uint8_t* __restrict__ ct, BatchGrid g,
                                                                 const uint32_t* __restrict__ mis_count) {
     if (*mis_count <= kMisfitCap) return;
-    const WorkItem wi = wl[blockIdx.x];
-    if (wi.tank == kNoTank) return;
-    const BatchTank& bt = T[wi.tank];
-    const StepParams P = bt.P;
-    int a;
-    uint32_t f;
-    if (!split_face(P, wi.start + threadIdx.x, a, f)) return;
-    const uint32_t fo = bt.face_off[a];
-    uint32_t d[3], i, j, k;
-    face_dims(P, a, d);
-    face_ijk(f, d, i, j, k);
-    float vv, ss;
-    face_epilogue(P, a, ct + bt.cell_off, i, j, k, g.num[a][fo + f], g.den[a][fo + f], vv, ss);
-    g.vel[a][fo + f] = vv;
-    g.saved[a][fo + f] = ss;
+    for (uint32_t w = blockIdx.x; w < nwl; w += gridDim.x) {
+        const WorkItem wi = wl[w];
+        if (wi.tank == kNoTank) continue;
+        const BatchTank& bt = T[wi.tank];
+        const StepParams P = bt.P;
+        int a;
+        uint32_t f;
+        if (!split_face(P, wi.start + threadIdx.x, a, f)) continue;
+        const uint32_t fo = bt.face_off[a];
+        uint32_t d[3], i, j, k;
+        face_dims(P, a, d);
+        face_ijk(f, d, i, j, k);
+        float vv, ss;
+        face_epilogue(P, a, ct + bt.cell_off, i, j, k, g.num[a][fo + f], g.den[a][fo + f], vv, ss);
+        g.vel[a][fo + f] = vv;
+        g.saved[a][fo + f] = ss;
+    }
 }
 
 hipError_t launch_batch_p2g_fallback(const BatchTank* T, const WorkItem* wl_part, uint32_t npart,
@@ -189,14 +198,20 @@ hipError_t launch_batch_p2g_fallback(const BatchTank* T, const WorkItem* wl_part
                                      const uint8_t* ct, const BatchGrid& g, const uint32_t* mis_count,
                                      hipStream_t s) {
     if (npart) {
+        const uint32_t gp = npart < kFallbackGrid ? npart : kFallbackGrid;
         if (bf16)
-            hipLaunchKernelGGL(kb_p2g_atomic<1>, dim3(npart), dim3(kBatchBlock), 0, s, T, wl_part, sorted, g, mis_count);
+            hipLaunchKernelGGL(kb_p2g_atomic<1>, dim3(gp), dim3(kBatchBlock), 0, s, T, wl_part, npart, sorted, g,
+                               mis_count);
         else
-            hipLaunchKernelGGL(kb_p2g_atomic<0>, dim3(npart), dim3(kBatchBlock), 0, s, T, wl_part, sorted, g, mis_count);
+            hipLaunchKernelGGL(kb_p2g_atomic<0>, dim3(gp), dim3(kBatchBlock), 0, s, T, wl_part, npart, sorted, g,
+                               mis_count);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (nface) hipLaunchKernelGGL(kb_face_epilogue, dim3(nface), dim3(kBatchBlock), 0, s, T, wl_face, ct, g, mis_count);
+    if (nface) {
+        const uint32_t gf = nface < kFallbackGrid ? nface : kFallbackGrid;
+        hipLaunchKernelGGL(kb_face_epilogue, dim3(gf), dim3(kBatchBlock), 0, s, T, wl_face, nface, ct, g, mis_count);
+    }
     return hipGetLastError();
 }
 
@@ -232,55 +247,87 @@ hipError_t launch_batch_div_cginit(const BatchTank* T, const WorkItem* wl, uint3
 }
 
 // ---------------------------------------------------------------------------
-// Group barrier + sum: like the single-tank grid_sum, over the workgroups of
-// one tank.  Arrival is a release RMW on the tank's counter; the last arrival
-// resets it and bumps the tank's generation word; the others poll the word
-// with relaxed loads and acquire once.  Every workgroup then sums the group's
-// published partials in slot order, so all hold the identical total.
-// Partials alternate between two halves by generation parity.
-template <int THREADS>
-__device__ bool group_sum(double v, const BatchCgWork& w, uint32_t tank, uint32_t base, uint32_t nb, uint32_t rank,
-                          double& out) {
-    __shared__ double s_w[THREADS / 64];
-    __shared__ double s_tree[THREADS];
-    __shared__ int s_abort;
-    __shared__ uint32_t s_gen;
-    const double bs = block_sum<THREADS>(v, s_w);
-    uint32_t* cnt = w.arrive + 32u * tank;
-    uint32_t* gen = w.gen + 32u * tank;
-    if (threadIdx.x == 0) {
-        const uint32_t g0 = ld_relaxed(gen);
-        w.partials[(size_t)(g0 & 1u) * gridDim.x + base + rank] = bs;
-        s_gen = g0;
-        s_abort = 0;
-        const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == nb - 1u) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const long long t0 = wall_clock64();
-            while (ld_relaxed(gen) == g0) {
-                if (ld_relaxed(w.abort_flag)) {
-                    s_abort = 1;
-                    break;
-                }
-                if (wall_clock64() - t0 > kSpinTimeoutTicks) {
-                    __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    s_abort = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-    }
-    __syncthreads();
-    if (s_abort) return false;
-    out = sum_partials<THREADS>(w.partials + (size_t)(s_gen & 1u) * gridDim.x + base, nb, s_tree);
+// Lock-step reductions of the batched solve.  Every workgroup of the launch
+// meets every other at one grid barrier per reduction (two-level arrival as
+// in the single-tank solver: a counter per group of workgroups with equal
+// blockIdx % 8, then one over the groups; release RMWs, one acquire for the
+// last arrival, relaxed polling of the generation word by the others).  Each
+// workgroup publishes its f64 partial in its slot and whether its tank is
+// still iterating; afterwards it sums the slots of its own tank,
+// [base, base + nb), in slot order (so every workgroup of a tank holds the
+// identical total) and ORs all flags.  The barriers of all tanks coincide, so
+// the L2 invalidation that each acquire performs falls between phases rather
+// than in the middle of other tanks' phases.  Partials and flags alternate
+// between two halves by generation parity.
+__device__ __forceinline__ bool arrive_grid(const BatchCgWork& w) {
+    const uint32_t G = gridDim.x;
+    const uint32_t grp = blockIdx.x & 7u;
+    const uint32_t in_grp = (G - grp + 7u) / 8u;
+    const uint32_t ngrp = G < 8u ? G : 8u;
+    uint32_t* gc = w.arrive + grp * 32u;
+    uint32_t* top = w.arrive + 8u * 32u;
+    const uint32_t prev = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != in_grp - 1u) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t prev2 = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev2 != ngrp - 1u) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
+template <int THREADS>
+__device__ bool lockstep_sum(double v, uint32_t flag, const BatchCgWork& w, uint32_t slot, uint32_t base, uint32_t nb,
+                             double& out, bool& any) {
+    __shared__ double s_w[THREADS / 64];
+    __shared__ double s_tree[THREADS];
+    __shared__ int s_last, s_abort;
+    __shared__ uint32_t s_gen;
+    const double bs = block_sum<THREADS>(v, s_w);
+    const uint32_t G = gridDim.x;
+    if (threadIdx.x == 0) {
+        const uint32_t g0 = ld_relaxed(w.gen);
+        const size_t half = (size_t)(g0 & 1u) * G;
+        w.partials[half + slot] = bs;
+        w.flags[half + slot] = flag;
+        s_gen = g0;
+        s_abort = 0;
+        s_last = arrive_grid(w) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_last) {
+        if (threadIdx.x == 0) __hip_atomic_store(w.gen, s_gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        const long long t0 = wall_clock64();
+        while (ld_relaxed(w.gen) == s_gen) {
+            if (ld_relaxed(w.abort_flag)) {
+                s_abort = 1;
+                break;
+            }
+            if (wall_clock64() - t0 > kSpinTimeoutTicks) {
+                __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (s_abort) return false;
+    const size_t half = (size_t)(s_gen & 1u) * G;
+    uint32_t f = 0;
+    for (uint32_t i = threadIdx.x; i < G; i += THREADS)
+        f |= __hip_atomic_load(&w.flags[half + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    any = __syncthreads_or(f != 0u) != 0;
+    out = nb ? sum_partials<THREADS>(w.partials + half + base, nb, s_tree) : 0.0;
+    return true;
+}
+
+// Workgroups of a tank take its slice of the liquid-tile list; workgroups
+// whose tank has converged (or padding workgroups, which have none) keep
+// meeting the barriers with zero partials until no tank iterates.
 template <int U>
 __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict__ T, const WorkItem* __restrict__ wl,
                                                       const float* __restrict__ b, float* __restrict__ p,
@@ -288,15 +335,20 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
                                                       float* __restrict__ d1, float* __restrict__ q, BatchCgWork w) {
     __shared__ uint32_t s_tiles[kBLdsTiles];
     const WorkItem wi = wl[blockIdx.x];
-    if (wi.tank == kNoTank) return;  // padding workgroup: in no group
-    const uint32_t tank = wi.tank;
+    const bool has_tank = wi.tank != kNoTank;
+    const uint32_t tank = has_tank ? wi.tank : 0u;
     const BatchTank& bt = T[tank];
     const StepParams P = bt.P;
     const uint32_t co = bt.cell_off, tile_off = bt.tile_off;
-    const uint32_t G = bt.cg_nb, rank = wi.start, base = bt.cg_base;
-    // The tank's liquid tiles are the slice [tb, te) of the compacted list.
-    const uint32_t tb = w.tile_scan[tile_off], te = w.tile_scan[tile_off + bt.ntiles];
-    const uint32_t nt = te - tb;
+    const uint32_t G = has_tank ? bt.cg_nb : 1u, rank = has_tank ? wi.start : 0u;
+    const uint32_t base = has_tank ? bt.cg_base : 0u, nb = has_tank ? bt.cg_nb : 0u;
+    const uint32_t slot = has_tank ? base + rank : wi.start;  // padding items carry their own slot
+    // The tank's liquid tiles are the slice [tb, tb + nt) of the compacted list.
+    uint32_t tb = 0, nt = 0;
+    if (has_tank) {
+        tb = w.tile_scan[tile_off];
+        nt = w.tile_scan[tile_off + bt.ntiles] - tb;
+    }
     const TileWalk tw = tile_walk(nt, rank, G, kBCgThreads / 64);
     const uint32_t s0 = (uint32_t)(((uint64_t)nt * rank) / G);
     const uint32_t s1 = (uint32_t)(((uint64_t)nt * (rank + 1u)) / G);
@@ -314,7 +366,7 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
     float* q_ = q + co;
 
     auto report = [&](double rs_init, double rs_final, double tr, uint32_t iters, uint32_t err) {
-        if (rank == 0 && threadIdx.x == 0) {
+        if (has_tank && rank == 0 && threadIdx.x == 0) {
             BatchCgState* st = w.state + tank;
             st->rs_init = rs_init;
             st->rs_final = rs_final;
@@ -327,7 +379,8 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<U>(P, rr_, tw, ti, tile_id);
     double rs0;
-    if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, rs0)) {
+    bool any = false;
+    if (!lockstep_sum<kBCgThreads>(acc, has_tank ? 1u : 0u, w, slot, base, nb, rs0, any)) {
         report(0.0, 0.0, 0.0, 0u, 1u);
         return;
     }
@@ -339,40 +392,48 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
     float* dprev = d0 + co;
     float* dnext = d1 + co;
     uint32_t iters = 0;
-    for (uint32_t it = 0; it < P.iters; ++it) {
-        ++iters;
+    bool active = has_tank;
+    for (uint32_t it = 0;; ++it) {
         const bool first = it == 0;
         float* dn_buf = first ? dprev : dnext;
         acc = 0.0;
-        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-            acc += phase_a_batch<U>(P, nm, rr_, dprev, dnext, q_, beta, first, tw, ti, tile_id);
+        if (active) {
+            ++iters;
+            for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                acc += phase_a_batch<U>(P, nm, rr_, dprev, dnext, q_, beta, first, tw, ti, tile_id);
+        }
         double dq;
-        if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, dq)) {
+        if (!lockstep_sum<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, dq, any)) {
             report(rs_init, rs_old, 0.0, iters, 1u);
             return;
         }
+        if (!any) break;  // every tank has finished (the same flags are seen by all)
         const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
         acc = 0.0;
-        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-            acc += phase_b_batch<U>(P, dn_buf, q_, pp, rr_, alpha, tw, ti, tile_id);
+        if (active)
+            for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                acc += phase_b_batch<U>(P, dn_buf, q_, pp, rr_, alpha, tw, ti, tile_id);
         double rr;
-        if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, rr)) {
+        if (!lockstep_sum<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, rr, any)) {
             report(rs_init, rs_old, 0.0, iters, 1u);
             return;
         }
-        beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
-        rs_old = rr;
-        if (tol > 0.0 && rr <= tol_sq * rs_init) break;
-        if (!first) {
-            float* t = dprev;
-            dprev = dnext;
-            dnext = t;
+        if (active) {
+            beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
+            rs_old = rr;
+            if ((tol > 0.0 && rr <= tol_sq * rs_init) || iters >= P.iters) {
+                active = false;
+            } else if (!first) {
+                float* t = dprev;
+                dprev = dnext;
+                dnext = t;
+            }
         }
     }
     acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<U>(P, nm, bb, pp, tw, ti, tile_id);
     double tr;
-    if (!group_sum<kBCgThreads>(acc, w, tank, base, G, rank, tr)) {
+    if (!lockstep_sum<kBCgThreads>(acc, 0u, w, slot, base, nb, tr, any)) {
         report(rs_init, rs_old, 0.0, iters, 1u);
         return;
     }
@@ -386,7 +447,7 @@ int batch_cg_max_blocks() {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     int lo = 1 << 30;
     // The smallest occupancy over the unroll variants, so every variant fits.
-    const void* fns[3] = {(const void*)kb_cg<4>, (const void*)kb_cg<2>, (const void*)kb_cg<8>};
+    const void* fns[3] = {(const void*)kb_cg<4>, (const void*)kb_cg<2>, (const void*)kb_cg<1>};
     for (const void* f : fns) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBCgThreads, 0) != hipSuccess) return 0;
@@ -400,7 +461,7 @@ hipError_t launch_batch_cg(const BatchTank* T, const WorkItem* wl, uint32_t nwl,
     if (nwl == 0) return hipSuccess;
     void* args[] = {(void*)&T, (void*)&wl, (void*)&b, (void*)&p, (void*)&r,
                     (void*)&d0, (void*)&d1, (void*)&q, (void*)&w};
-    const void* f = cg_unroll == 1 ? (const void*)kb_cg<2> : (cg_unroll == 2 ? (const void*)kb_cg<8> : (const void*)kb_cg<4>);
+    const void* f = cg_unroll == 1 ? (const void*)kb_cg<2> : (cg_unroll == 2 ? (const void*)kb_cg<1> : (const void*)kb_cg<4>);
     return hipLaunchCooperativeKernel(f, dim3(nwl), dim3(kBCgThreads), args, 0, s);
 }
 

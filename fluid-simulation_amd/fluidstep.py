@@ -38,6 +38,11 @@ EXPORTED = (
     "fs_read_particles", "fs_read_faces", "fs_read_cells_f32", "fs_read_cells_u32",
     "fs_read_permutation", "fs_get_stats", "fs_get_params", "fs_particle_count",
     "fs_buffer_bytes", "fs_last_error", "fs_version",
+    "fs_batch_create", "fs_batch_destroy", "fs_batch_step", "fs_batch_synchronize",
+    "fs_batch_set_timing", "fs_batch_autotune", "fs_batch_set_tuning", "fs_batch_tank_count",
+    "fs_batch_buffer_bytes", "fs_batch_read_particles", "fs_batch_read_faces",
+    "fs_batch_read_cells_f32", "fs_batch_read_cells_u32", "fs_batch_read_permutation",
+    "fs_batch_get_stats",
 )
 
 
@@ -101,6 +106,24 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """fs_batch_tuning: menu indices of the tuned tile shapes."""
+    _fields_ = [
+        ("gather_brick", C.c_uint32), ("sort_tile", C.c_uint32), ("cg_unroll", C.c_uint32),
+        ("cg_blocks", C.c_uint32), ("ms_gather", C.c_double), ("ms_sort", C.c_double), ("ms_cg", C.c_double),
+    ]
+    BRICKS = ("16x8x8", "8x8x8", "32x4x4")
+    SORT_TILES = (4096, 2048, 8192)
+    CG_UNROLL = (4, 2, 1)
+
+    def as_dict(self):
+        return {
+            "gather_brick": self.BRICKS[self.gather_brick], "sort_tile": self.SORT_TILES[self.sort_tile],
+            "cg_unroll": self.CG_UNROLL[self.cg_unroll], "cg_blocks": self.cg_blocks,
+            "ms_gather": self.ms_gather, "ms_sort": self.ms_sort, "ms_cg": self.ms_cg,
+        }
+
+
 _lib = None
 
 
@@ -145,6 +168,24 @@ def lib():
         L.fs_buffer_bytes.restype = C.c_uint64
         L.fs_last_error.restype = C.c_char_p
         L.fs_version.restype = C.c_char_p
+        L.fs_batch_create.argtypes = [C.POINTER(Params), C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_uint32),
+                                      C.c_uint32, C.POINTER(vp)]
+        L.fs_batch_destroy.argtypes = [vp]
+        L.fs_batch_step.argtypes = [vp, C.c_uint32]
+        L.fs_batch_synchronize.argtypes = [vp]
+        L.fs_batch_set_timing.argtypes = [vp, C.c_int]
+        L.fs_batch_autotune.argtypes = [vp, C.POINTER(Tuning)]
+        L.fs_batch_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.fs_batch_tank_count.argtypes = [vp]
+        L.fs_batch_tank_count.restype = C.c_uint32
+        L.fs_batch_buffer_bytes.argtypes = [vp]
+        L.fs_batch_buffer_bytes.restype = C.c_uint64
+        L.fs_batch_read_particles.argtypes = [vp, C.c_uint32, vp, vp]
+        L.fs_batch_read_faces.argtypes = [vp, C.c_uint32, C.c_int, C.c_int, vp]
+        L.fs_batch_read_cells_f32.argtypes = [vp, C.c_uint32, C.c_int, vp]
+        L.fs_batch_read_cells_u32.argtypes = [vp, C.c_uint32, C.c_int, vp]
+        L.fs_batch_read_permutation.argtypes = [vp, C.c_uint32, vp]
+        L.fs_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -330,3 +371,111 @@ class FluidStep:
 
     def buffer_bytes(self) -> int:
         return int(lib().fs_buffer_bytes(self._h))
+
+
+class FluidBatch:
+    """Many tanks stepped together (fs_batch_*): one launch per kernel phase
+    over all tanks, the substep replayed from a captured hipGraph."""
+
+    def __init__(self, params_list, pos_list, vel_list=None):
+        self.n_tanks = len(params_list)
+        assert len(pos_list) == self.n_tanks
+        self._pos = [np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3) for p in pos_list]
+        self._vel = None if vel_list is None else [
+            np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3) for v in vel_list]
+        prm = (Params * self.n_tanks)(*params_list)
+        pp = (C.c_void_p * self.n_tanks)(*[p.ctypes.data for p in self._pos])
+        vp_ = None if self._vel is None else (C.c_void_p * self.n_tanks)(*[v.ctypes.data for v in self._vel])
+        counts = (C.c_uint32 * self.n_tanks)(*[len(p) for p in self._pos])
+        h = C.c_void_p()
+        _check(lib().fs_batch_create(prm, pp, vp_, counts, self.n_tanks, C.byref(h)))
+        self._h = h
+        self.params = list(params_list)
+        self.counts = [len(p) for p in self._pos]
+        self._pos = self._vel = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fs_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, substeps: int = 1):
+        _check(lib().fs_batch_step(self._h, C.c_uint32(substeps)))
+
+    def synchronize(self):
+        _check(lib().fs_batch_synchronize(self._h))
+
+    def set_timing(self, on: bool):
+        _check(lib().fs_batch_set_timing(self._h, int(on)))
+
+    def autotune(self) -> dict:
+        t = Tuning()
+        _check(lib().fs_batch_autotune(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, gather_brick=0, sort_tile=0, cg_unroll=0, cg_blocks=0):
+        t = Tuning(gather_brick, sort_tile, cg_unroll, cg_blocks, 0.0, 0.0, 0.0)
+        _check(lib().fs_batch_set_tuning(self._h, C.byref(t)))
+
+    def buffer_bytes(self) -> int:
+        return int(lib().fs_batch_buffer_bytes(self._h))
+
+    def _dims(self, t):
+        p = self.params[t]
+        return p.nx, p.ny, p.nz
+
+    def particles(self, t):
+        n = self.counts[t]
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        _check(lib().fs_batch_read_particles(self._h, t, _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def faces(self, t, which=FACE_VELOCITY):
+        out = []
+        for a in range(3):
+            d = list(self._dims(t))
+            d[a] += 1
+            f = np.zeros(d[0] * d[1] * d[2], np.float32)
+            _check(lib().fs_batch_read_faces(self._h, t, a, which, _ptr(f)))
+            out.append(f)
+        return out
+
+    def _cells(self, t, which, dtype, fn):
+        nx, ny, nz = self._dims(t)
+        out = np.zeros(nx * ny * nz, dtype)
+        _check(fn(self._h, t, which, _ptr(out)))
+        return out
+
+    def pressure(self, t):
+        return self._cells(t, CELL_PRESSURE, np.float32, lib().fs_batch_read_cells_f32)
+
+    def divergence(self, t):
+        return self._cells(t, CELL_DIVERGENCE, np.float32, lib().fs_batch_read_cells_f32)
+
+    def cell_type(self, t):
+        return self._cells(t, CELL_TYPE, np.uint32, lib().fs_batch_read_cells_u32)
+
+    def occupancy(self, t):
+        return self._cells(t, CELL_OCCUPANCY, np.uint32, lib().fs_batch_read_cells_u32)
+
+    def cell_offset(self, t):
+        return self._cells(t, CELL_OFFSET, np.uint32, lib().fs_batch_read_cells_u32)
+
+    def permutation(self, t):
+        p = np.zeros(self.counts[t], np.uint32)
+        _check(lib().fs_batch_read_permutation(self._h, t, _ptr(p)))
+        return p
+
+    def stats(self, t) -> dict:
+        s = Stats()
+        _check(lib().fs_batch_get_stats(self._h, t, C.byref(s)))
+        return s.as_dict()

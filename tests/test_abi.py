@@ -149,6 +149,29 @@ def test_invalid_arguments_are_rejected_before_the_device(fsmod):
     assert e.value.code == -2
 
 
+def test_batch_rejects_what_it_does_not_run_before_the_device(fsmod):
+    """A batch runs the fused, mixed-precision, sort-every-substep step with
+    one velocity storage; other requests fail with FS_ERR_UNSUPPORTED (and
+    bad tanks with their single-tank codes) before any device call."""
+    pos = np.full((4, 3), 0.5, np.float32)
+    kw = dict(engine=1, precision=1, particle_sort=1, particle_sort_period=1)
+    ok = fsm.make_params(16, 16, 16, 1 / 16, **kw)
+    cases = [
+        (fsm.make_params(16, 16, 16, 1 / 16, **dict(kw, engine=0, precision=0)), -2),
+        (fsm.make_params(16, 16, 16, 1 / 16, **dict(kw, particle_sort=0)), -2),
+        (fsm.make_params(16, 16, 16, 1 / 16, **dict(kw, particle_sort_period=4)), -2),
+        (fsm.make_params(16, 16, 16, 1 / 16, **dict(kw, velocity_storage=1)), -2),
+        (fsm.make_params(2, 16, 16, 1 / 16, **kw), -1),
+    ]
+    for second, code in cases:
+        with pytest.raises(fsm.FluidStepError) as e:
+            fsm.FluidBatch([ok, second], [pos, pos])
+        assert e.value.code == code
+    with pytest.raises(fsm.FluidStepError) as e:
+        fsm.FluidBatch([], [])
+    assert e.value.code == -1
+
+
 def test_workload_particle_counts():
     assert len(scenes.dam_break_32()["pos"]) == 77824            # 16 x 19 x 32 x 8
     assert len(scenes.control_64()["pos"]) == 524288             # 32 x 32 x 64 x 8
