@@ -162,6 +162,129 @@ def batch_bench(fsm, scenes, gpu_params, args):
     return line
 
 
+def adjoint_report(fsm, scenes, gpu_params, steps=60, every=10, h=1e-3, fp64=True):
+    """Differentiable control at 64^3: gradient of |centroid - (0.7,0.3,0.5)|^2
+    after `steps` substeps w.r.t. the initial particle velocities and gravity_y,
+    from the checkpointed forward + reverse pass on the GPU; checked against
+    central differences of the GPU forward (h) along uniform velocity shifts
+    and gravity, and against the FP64 reverse pass of the restatement."""
+    sc = scenes.control_64()
+    target = np.array([0.7, 0.3, 0.5])
+    params = gpu_params(sc, fsm.ENGINE_FUSED)
+
+    def upstream(pos):
+        c = pos.astype(np.float64).mean(0)
+        return np.tile(2.0 * (c - target) / len(pos), (len(pos), 1))
+
+    def loss(pos):
+        c = pos.astype(np.float64).mean(0)
+        return float(((c - target) ** 2).sum())
+
+    t = fsm.FluidStep(params, sc["pos"], sc["vel"])
+    t.forward_checkpointed(1, 1)  # warm-up of every kernel
+    t.close()
+    t = fsm.FluidStep(params, sc["pos"], sc["vel"])
+    t0 = time.perf_counter()
+    t.forward_checkpointed(steps, every)
+    t.synchronize()
+    fwd_s = time.perf_counter() - t0
+    pos, _ = t.particles()
+    L0 = loss(pos)
+    t0 = time.perf_counter()
+    gx, gv, gg = t.adjoint(upstream(pos), None)
+    bwd_s = time.perf_counter() - t0
+    t.close()
+
+    def L_at(vel, gy):
+        p = fsm.Params()
+        import ctypes
+        ctypes.pointer(p)[0] = params
+        p.gravity[1] = gy
+        tt = fsm.FluidStep(p, sc["pos"], vel.astype(np.float32))
+        tt.step(steps)
+        pp, _ = tt.particles()
+        tt.close()
+        return loss(pp)
+
+    checks = {}
+    v0 = sc["vel"].astype(np.float64)
+    for axis, name in enumerate("xyz"):
+        d = np.zeros_like(v0)
+        d[:, axis] = 1.0
+        fd = (L_at(v0 + h * d, -9.81) - L_at(v0 - h * d, -9.81)) / (2 * h)
+        ad = float((gv.astype(np.float64) * d).sum())
+        checks[f"vel_shift_{name}"] = {"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / max(abs(fd), 1e-30)}
+    fd = (L_at(v0, -9.81 + h) - L_at(v0, -9.81 - h)) / (2 * h)
+    checks["gravity_y"] = {"fd": fd, "adjoint": float(gg[1]), "rel_err": abs(fd - gg[1]) / max(abs(fd), 1e-30)}
+    report = {
+        "workload": "control_64", "particles": len(sc["pos"]), "steps": steps, "checkpoint_every": every,
+        "loss": L0, "gpu_forward_checkpointed_s": fwd_s, "gpu_adjoint_s": bwd_s,
+        "fd_h": h, "fd_checks_gpu_forward": checks,
+        "grad_norms": {"vel": float(np.linalg.norm(gv)), "pos": float(np.linalg.norm(gx)), "gravity": gg.tolist()},
+    }
+    if fp64:
+        import oracle
+
+        import helpers
+
+        rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        P = helpers.oracle_params(sc, 1)
+        # (1) The FP64 reverse pass along the GPU's own trajectory: the GPU
+        # forward's input state of every substep (read-out order, warm-start
+        # pressure), then the FP64 adjoint of each substep from that state.
+        t0 = time.perf_counter()
+        tt = fsm.FluidStep(params, sc["pos"], sc["vel"])
+        states = []
+        for _ in range(steps):
+            p_, v_ = tt.particles()
+            states.append((p_, v_, tt.pressure()))
+            tt.step(1)
+        pf, _ = tt.particles()
+        tt.close()
+        T = oracle.AdjointTape.from_states(P, states)
+        del states
+        tgx, tgv, tgg = T.backward(upstream(pf), None)
+        T.close()
+        report["fp64_on_gpu_trajectory"] = {
+            "seconds": time.perf_counter() - t0,
+            "rel_l2_vel_grad": rel(gv.astype(np.float64), tgv), "rel_l2_pos_grad": rel(gx.astype(np.float64), tgx),
+            "gravity": tgg.tolist(), "rel_err_gravity": float(np.linalg.norm(gg - tgg) / max(np.linalg.norm(tgg), 1e-300)),
+        }
+        # (2) The FP64 forward and reverse pass on their own trajectory, and
+        # central differences (h) of the FP64 forward with the base run's
+        # cell types and occupancies replayed (the function the adjoint
+        # differentiates: the plain forward's difference also carries the
+        # jumps of surface cells that flip between the +h and -h runs).
+        t0 = time.perf_counter()
+        T = oracle.AdjointTape(P, sc["pos"].astype(np.float64), v0, steps=steps, every=every, f64=True, record=True)
+        ogx, ogv, ogg = T.backward(upstream(T.pos), None)
+        frozen = {}
+
+        def Lf(vel, gy):
+            Pg = helpers.oracle_params(sc, 1)
+            Pg.grav[1] = gy
+            pos_, _, _ = oracle.run_frozen(Pg, sc["pos"].astype(np.float64), vel, T.ct_steps, T.occ_steps)
+            return loss(pos_)
+
+        for axis, name in enumerate("xyz"):
+            d = np.zeros_like(v0)
+            d[:, axis] = 1.0
+            fd = (Lf(v0 + h * d, -9.81) - Lf(v0 - h * d, -9.81)) / (2 * h)
+            ad = float((ogv * d).sum())
+            frozen[f"vel_shift_{name}"] = {"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / max(abs(fd), 1e-30)}
+        fd = (Lf(v0, -9.81 + h) - Lf(v0, -9.81 - h)) / (2 * h)
+        frozen["gravity_y"] = {"fd": fd, "adjoint": float(ogg[1]), "rel_err": abs(fd - ogg[1]) / max(abs(fd), 1e-30)}
+        report["fp64_reference"] = {
+            "seconds": time.perf_counter() - t0, "threads": oracle.num_threads(), "loss": loss(T.pos),
+            "frozen_decision_fd": frozen,
+            "gpu_vs_fp64_own_trajectories": {
+                "rel_l2_vel_grad": rel(gv.astype(np.float64), ogv), "rel_l2_pos_grad": rel(gx.astype(np.float64), ogx),
+                "gravity_y": float(ogg[1]), "rel_err_gravity_y": abs(gg[1] - ogg[1]) / max(abs(ogg[1]), 1e-30)},
+        }
+        T.close()
+    return report
+
+
 def _track(ids, perm):
     """Compose the sort permutation into the particle-identity array."""
     return ids[perm] if perm is not None else ids
@@ -256,6 +379,9 @@ def main():
                     help="instead of timing, report each GPU precision mode's error against the FP64 "
                          "restatement after STEPS steps (and one-step errors from its states)")
     ap.add_argument("--teacher-states", type=int, default=3)
+    ap.add_argument("--adjoint-report", type=int, default=0, metavar="STEPS",
+                    help="instead of timing, run the differentiable-control check at 64^3 over STEPS substeps")
+    ap.add_argument("--no-fp64", action="store_true", help="adjoint report: skip the FP64 reference gradient")
     ap.add_argument("--workload", default="high_density_128",
                     choices=["high_density_128", "large_splash_256", "dam_break_32", "control_64", "batched_tanks"],
                     help="the metric is quoted on high_density_128; the others are for per-kernel profiles "
@@ -278,6 +404,11 @@ def main():
 
     if args.precision_report:
         print(json.dumps(precision_report(args.precision_report, args.teacher_states)), flush=True)
+        return
+
+    if args.adjoint_report:
+        print(json.dumps(adjoint_report(fsm, scenes, gpu_params, steps=args.adjoint_report, fp64=not args.no_fp64)),
+              flush=True)
         return
 
     if args.workload == "batched_tanks":

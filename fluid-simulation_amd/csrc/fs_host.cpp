@@ -18,6 +18,7 @@
 #include "fluidstep.h"
 #include "fs_kernels.h"
 #include "fs_host_common.h"
+#include "fs_tank.h"
 
 using namespace fs;
 
@@ -133,75 +134,10 @@ double value_of(const std::vector<double>& v, const char* id) {
 
 }  // namespace
 
+std::vector<double> fs_default_settings() { return default_values(); }
+
 struct fs_registry {
     std::vector<double> values = default_values();
-};
-
-// ---------------------------------------------------------------------------
-// Tank
-struct fs_tank {
-    fs_params prm{};
-    StepParams P{};
-    hipStream_t stream = nullptr;
-    uint32_t N = 0;
-    int bf16 = 0;
-    std::vector<double> settings = default_values();
-
-    std::vector<float> init_pos, init_vel;  // GpuFluid::initial (reset source)
-
-    ParticleSoA side[2]{};
-    int cur = 0;
-    uint32_t *keys = nullptr, *ktmp0 = nullptr, *ktmp1 = nullptr, *vtmp = nullptr, *perm = nullptr;
-    uint32_t *radix_ws = nullptr, *scan_ws = nullptr;
-    uint32_t *occ = nullptr, *cell_offset = nullptr;
-    uint8_t* ct = nullptr;
-    int32_t *num[3] = {}, *den[3] = {};
-    float *vel[3] = {}, *saved[3] = {};
-    float *div = nullptr, *b = nullptr, *p = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr, *q = nullptr;
-    float* scal = nullptr;      // reference CG scalars [8]
-    float* partials = nullptr;  // reference CG partials
-    uint32_t* counters = nullptr;  // [0] liquid cells, [1] misfits, [2] CG iterations (reference engine)
-    uint32_t* mis_list = nullptr;  // indices (pre-sort side) of the first kMisfitCap misfit particles
-    CgWork cg{};
-    int cg_blocks = 0;        // persistent form
-    int cg_mode = 0;          // 0 persistent kernel, 1 two kernels per iteration
-    int cg_multi_blocks = 0;  // multi-kernel form
-    int cg_use_graph = 1;     // replay the multi-kernel solve from a captured hipGraph
-    hipGraphExec_t cg_exec = nullptr;
-
-    uint64_t tick = 0;  // sort tick (advance_sort_tick, src/gpu/fluid.rs:1491-1498)
-    uint64_t substeps = 0;
-    int sorted_last = 0, gather_last = 0, perm_valid = 0;
-    int timing = 0;
-    hipEvent_t ev[5] = {};
-    double ms[5] = {0, 0, 0, 0, 0};
-    uint64_t bytes = 0;
-
-    std::vector<void*> allocs;
-
-    template <typename T>
-    hipError_t alloc(T*& ptr, size_t count) {
-        void* p_ = nullptr;
-        size_t nb = count * sizeof(T);
-        if (nb == 0) nb = 16;
-        hipError_t e = hipMalloc(&p_, nb);
-        if (e != hipSuccess) return e;
-        allocs.push_back(p_);
-        bytes += nb;
-        ptr = (T*)p_;
-        return hipSuccess;
-    }
-    void drop_graph() {
-        if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
-        cg_exec = nullptr;
-    }
-    ~fs_tank() {
-        drop_graph();
-        for (void* a : allocs) (void)hipFree(a);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
 };
 
 namespace fs {
@@ -382,7 +318,11 @@ int substep_reference(fs_tank* t, bool sort_now) {
 }
 
 // ---- one substep, fused engine ----
-int substep_fused(fs_tank* t, bool sort_now) {
+}  // namespace
+
+namespace fs {
+
+int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     const StepParams& P = t->P;
     hipStream_t s = t->stream;
     if (t->timing) FS_HIP(hipEventRecord(t->ev[0], s));
@@ -448,6 +388,7 @@ int substep_fused(fs_tank* t, bool sort_now) {
     }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[2], s));
     FS_HIP(launch_gradient_enforce(P, t->p, t->ct, t->vel, s));
+    if (stop_before_g2p) return FS_OK;
     FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
     if (t->timing) FS_HIP(hipEventRecord(t->ev[3], s));
     return FS_OK;
@@ -483,6 +424,10 @@ int do_step(fs_tank* t, uint32_t substeps) {
     }
     return FS_OK;
 }
+
+}  // namespace fs
+
+namespace {
 
 int check_tank(const fs_tank* t) {
     if (!t) return fail(FS_ERR_INVALID_ARGUMENT, "null tank handle");

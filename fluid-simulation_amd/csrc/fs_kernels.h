@@ -147,4 +147,35 @@ hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16,
 hipError_t launch_impulse(const StepParams& P, const ParticleSoA& live, int bf16, float dx, float dy,
                           float dz, hipStream_t s);
 
+// ---- k_adjoint.hip: reverse pass of one fused substep ----
+struct AdjFaces {
+    const float* vel[3];    // final face velocities of the forward substep
+    const float* saved[3];  // saved (normalized P2G) velocities
+    const int32_t* den[3];  // P2G weight sums (fixed point)
+    float* vbar[3];         // adjoint of the final face velocities (accumulated by k_adj_g2p)
+    float* sbar[3];         // adjoint of the saved velocities
+    float* numbar[3];       // adjoint of sum w v per face
+    float* denbar[3];       // adjoint of sum w per face
+};
+struct AdjParticles {
+    float* gx[3];
+    float* gv[3];
+};
+struct AdjParticlesIn {
+    const float* gx[3];
+    const float* gv[3];
+};
+hipError_t launch_adj_g2p(const StepParams& P, const ParticleSoA& sorted, const AdjFaces& faces,
+                          const AdjParticles& out, const AdjParticlesIn& in, hipStream_t s);
+hipError_t launch_adj_pbar(const StepParams& P, const uint8_t* ct, const AdjFaces& faces, float* pbar,
+                           hipStream_t s);
+hipError_t launch_adj_cg_init(const StepParams& P, const uint16_t* nmask, const float* pbar, float* rhs, float* z,
+                              float* r, float* d, hipStream_t s);
+hipError_t launch_adj_faces(const StepParams& P, const uint8_t* ct, const float* z, const AdjFaces& faces,
+                            double* grav, hipStream_t s);
+hipError_t launch_adj_p2g(const StepParams& P, const ParticleSoA& sorted, const AdjFaces& faces,
+                          const AdjParticles& out, hipStream_t s);
+hipError_t launch_adj_unsort(uint32_t n, const uint32_t* perm, const AdjParticles& sorted,
+                             const AdjParticles& in_order, hipStream_t s);
+
 }  // namespace fs

@@ -42,7 +42,7 @@ EXPORTED = (
     "fs_batch_set_timing", "fs_batch_autotune", "fs_batch_set_tuning", "fs_batch_tank_count",
     "fs_batch_buffer_bytes", "fs_batch_read_particles", "fs_batch_read_faces",
     "fs_batch_read_cells_f32", "fs_batch_read_cells_u32", "fs_batch_read_permutation",
-    "fs_batch_get_stats",
+    "fs_batch_get_stats", "fs_forward_checkpointed", "fs_step_adjoint",
 )
 
 
@@ -186,6 +186,8 @@ def lib():
         L.fs_batch_read_cells_u32.argtypes = [vp, C.c_uint32, C.c_int, vp]
         L.fs_batch_read_permutation.argtypes = [vp, C.c_uint32, vp]
         L.fs_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
+        L.fs_forward_checkpointed.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.fs_step_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -371,6 +373,22 @@ class FluidStep:
 
     def buffer_bytes(self) -> int:
         return int(lib().fs_buffer_bytes(self._h))
+
+    # differentiation
+    def forward_checkpointed(self, steps: int, every: int = 10):
+        """Run `steps` substeps keeping a snapshot every `every` substeps."""
+        _check(lib().fs_forward_checkpointed(self._h, C.c_uint32(steps), C.c_uint32(every)))
+
+    def adjoint(self, gpos=None, gvel=None):
+        """Reverse pass of the last forward_checkpointed: dL/d(final pos, vel)
+        (read-out order) -> (dL/d initial pos, dL/d initial vel, dL/d gravity)."""
+        gp = None if gpos is None else np.ascontiguousarray(gpos, dtype=np.float32).reshape(-1, 3)
+        gv = None if gvel is None else np.ascontiguousarray(gvel, dtype=np.float32).reshape(-1, 3)
+        opos = np.zeros((self.n, 3), np.float32)
+        ovel = np.zeros((self.n, 3), np.float32)
+        grav = np.zeros(3, np.float64)
+        _check(lib().fs_step_adjoint(self._h, _ptr(gp), _ptr(gv), _ptr(opos), _ptr(ovel), _ptr(grav)))
+        return opos, ovel, grav
 
 
 class FluidBatch:

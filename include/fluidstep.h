@@ -231,6 +231,29 @@ FS_API int fs_batch_read_cells_u32(fs_batch* b, uint32_t tank, int which, uint32
 FS_API int fs_batch_read_permutation(fs_batch* b, uint32_t tank, uint32_t* perm);
 FS_API int fs_batch_get_stats(fs_batch* b, uint32_t tank, fs_stats* out);
 
+/* ---- differentiation ----
+ * Reverse-mode gradients of a loss of the final particle state with respect
+ * to the initial particle state and the gravity vector.  The reference has no
+ * differentiation; what is differentiated is its step, GpuContext::step
+ * (src/gpu/mod.rs:950-1033) as run by the fused engine.  Conventions: the
+ * sort order, cell types, stencils, wet/solid faces and the CFL and wall-clamp
+ * branches of the forward are held fixed; fixed-point rounding of P2G counts
+ * as the identity; the pressure solve counts as exact (its adjoint is a CG
+ * solve with the same operator, to a 1e-6 relative residual).  Needs
+ * FS_ENGINE_FUSED, FS_PRECISION_MIXED, FS_VELOCITY_F32, particles sorted
+ * every substep and wall_friction 0 (FS_ERR_UNSUPPORTED otherwise). */
+/* From the tank's live state, run `steps` substeps keeping a snapshot
+ * (particles + warm-start pressure) every `every` substeps. */
+FS_API int fs_forward_checkpointed(fs_tank* t, uint32_t steps, uint32_t every);
+/* Reverse pass over the last fs_forward_checkpointed.  gpos_final /
+ * gvel_final: dL/d(final positions / velocities), xyz per particle in the
+ * order fs_read_particles gives after the forward (NULL = zero).  Outputs
+ * (nullable): dL/d(initial positions / velocities) in the order of the state
+ * the forward started from, and dL/d(gravity) (3 doubles).  Recomputes each
+ * checkpoint interval; the tank is left in the forward's final state. */
+FS_API int fs_step_adjoint(fs_tank* t, const float* gpos_final, const float* gvel_final, float* gpos_init,
+                           float* gvel_init, double* gravity_grad);
+
 #ifdef __cplusplus
 }
 #endif

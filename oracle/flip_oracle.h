@@ -134,6 +134,26 @@ int orc_run_f64(const orc_params *P, uint32_t n, double *px, double *py, double 
                 double *vx, double *vy, double *vz, double *pressure, uint32_t steps,
                 uint32_t sort_period, uint32_t tick0, orc_outputs_f64 *out);
 
+/* Reverse-mode gradient of `steps` substeps (sort every substep), see
+ * flip_oracle_adjoint.h.  adj_forward runs the checkpointed forward in place
+ * (pos/vel/pressure end in the final state) and returns a tape;
+ * adj_backward turns dL/d(final state) in gpos/gvel into dL/d(initial state)
+ * and fills grav with dL/d(gravity). */
+void *orc_adj_forward_f32(const orc_params *P, uint32_t n, float *const pos[3], float *const vel[3],
+                          float *pressure, uint32_t steps, uint32_t every, uint32_t *rec_ct, uint32_t *rec_occ);
+int orc_adj_backward_f32(void *tape, float *const gpos[3], float *const gvel[3], double grav[3]);
+void orc_adj_free_f32(void *tape);
+void orc_run_frozen_f32(const orc_params *P, uint32_t n, float *const pos[3], float *const vel[3], float *pressure,
+                        uint32_t steps, const uint32_t *ct_steps, const uint32_t *occ_steps);
+void *orc_adj_tape_from_states_f32(const orc_params *P, uint32_t n, uint32_t steps, const float *states);
+void *orc_adj_forward_f64(const orc_params *P, uint32_t n, double *const pos[3], double *const vel[3],
+                          double *pressure, uint32_t steps, uint32_t every, uint32_t *rec_ct, uint32_t *rec_occ);
+int orc_adj_backward_f64(void *tape, double *const gpos[3], double *const gvel[3], double grav[3]);
+void orc_adj_free_f64(void *tape);
+void orc_run_frozen_f64(const orc_params *P, uint32_t n, double *const pos[3], double *const vel[3],
+                        double *pressure, uint32_t steps, const uint32_t *ct_steps, const uint32_t *occ_steps);
+void *orc_adj_tape_from_states_f64(const orc_params *P, uint32_t n, uint32_t steps, const double *states);
+
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 

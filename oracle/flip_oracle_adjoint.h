@@ -1,0 +1,477 @@
+/*
+ * flip_oracle_adjoint.h — reverse-mode gradient of the restated step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see flip_oracle.h): the reference gradient that
+ * the HIP adjoint (fluid-simulation_amd/csrc/k_adjoint.hip) is checked
+ * against.  Included at the end of flip_oracle_body.h, so it exists in R =
+ * float and R = double; the double build is the FP64 reference gradient.
+ *
+ * The reference itself has no differentiation.  What is differentiated is
+ * the substep of FN(run) with a sort every substep (src/gpu/mod.rs:993-1033),
+ * under these conventions, shared with the HIP adjoint:
+ *   - the sort order, cell types, wet/solid/gradient faces, the static-wall
+ *     filter of the G2P gather, the CFL branch and the wall-clamp branch of
+ *     the forward are held fixed;
+ *   - fixed-point rounding of the P2G terms is the identity (num = sum w v,
+ *     den = sum w);
+ *   - the pressure solve is exact: b_bar = A^-1 p_bar, solved here by CG in R
+ *     to a relative residual of 1e-13 (double) / 1e-6 (float);
+ *   - wall_friction is 0.
+ * Checkpointing mirrors the HIP driver: a snapshot every `every` substeps,
+ * each interval recomputed during the reverse pass.
+ */
+
+typedef struct {
+    orc_params P;
+    uint32_t n, C, fc[3], steps, every, K;
+    real *ckpt;    /* K x (6n + C): pos xyz, vel xyz, pressure */
+    real *itape;   /* every x (6n + C) */
+} adj_tape;
+
+static size_t adj_snap_len(const adj_tape *T) { return 6 * (size_t)T->n + T->C; }
+
+static void adj_save(const adj_tape *T, real *dst, real *const pos[3], real *const vel[3], const real *pr) {
+    const size_t n = T->n;
+    for (int a = 0; a < 3; ++a) {
+        memcpy(dst + a * n, pos[a], sizeof(real) * n);
+        memcpy(dst + (3 + a) * n, vel[a], sizeof(real) * n);
+    }
+    memcpy(dst + 6 * n, pr, sizeof(real) * T->C);
+}
+
+static void adj_load(const adj_tape *T, const real *src, real *const pos[3], real *const vel[3], real *pr) {
+    const size_t n = T->n;
+    for (int a = 0; a < 3; ++a) {
+        memcpy(pos[a], src + a * n, sizeof(real) * n);
+        memcpy(vel[a], src + (3 + a) * n, sizeof(real) * n);
+    }
+    memcpy(pr, src + 6 * n, sizeof(real) * T->C);
+}
+
+typedef struct {
+    uint32_t *occ, *ct, *keys, *perm;
+    real *tmp, *div;
+    ACC *num[3], *den[3];
+    real *vel[3], *saved[3];
+} adj_ws;
+
+static void adj_ws_alloc(const adj_tape *T, adj_ws *w) {
+    const size_t n = T->n ? T->n : 1, C = T->C;
+    w->occ = (uint32_t *)malloc(sizeof(uint32_t) * C);
+    w->ct = (uint32_t *)malloc(sizeof(uint32_t) * C);
+    w->keys = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    w->perm = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    w->tmp = (real *)malloc(sizeof(real) * n);
+    w->div = (real *)malloc(sizeof(real) * C);
+    for (int a = 0; a < 3; ++a) {
+        w->num[a] = (ACC *)malloc(sizeof(ACC) * T->fc[a]);
+        w->den[a] = (ACC *)malloc(sizeof(ACC) * T->fc[a]);
+        w->vel[a] = (real *)malloc(sizeof(real) * T->fc[a]);
+        w->saved[a] = (real *)malloc(sizeof(real) * T->fc[a]);
+    }
+}
+
+static void adj_ws_free(adj_ws *w) {
+    free(w->occ); free(w->ct); free(w->keys); free(w->perm); free(w->tmp); free(w->div);
+    for (int a = 0; a < 3; ++a) {
+        free(w->num[a]); free(w->den[a]); free(w->vel[a]); free(w->saved[a]);
+    }
+}
+
+/* The substep up to G2P (sort every substep): particles end sorted, w holds
+ * the perm, cell types, P2G sums, saved and final face velocities.  With
+ * frozen_ct / frozen_occ (C entries each) the substep uses those cell types
+ * and occupancies instead of its own: the forward with the discrete
+ * decisions of another run, which the adjoint differentiates. */
+static void adj_prefix_frozen(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pr,
+                              adj_ws *w, const uint32_t *frozen_ct, const uint32_t *frozen_occ) {
+    const uint32_t C = P->nx * P->ny * P->nz;
+    orc_stats st;
+    memset(&st, 0, sizeof st);
+    if (!P->warm_start) memset(pr, 0, sizeof(real) * C);
+    FN(histogram)(P, n, pos[0], pos[1], pos[2], w->occ);
+    for (uint32_t p = 0; p < n; ++p) w->keys[p] = cell_key(P, pos[0][p], pos[1][p], pos[2][p]);
+    orc_stable_permutation(n, C, w->keys, w->perm);
+    for (int a = 0; a < 3; ++a) {
+        permute(n, w->perm, pos[a], w->tmp);
+        permute(n, w->perm, vel[a], w->tmp);
+    }
+    if (frozen_occ) memcpy(w->occ, frozen_occ, sizeof(uint32_t) * C);
+    if (frozen_ct) memcpy(w->ct, frozen_ct, sizeof(uint32_t) * C);
+    else classify(P, w->occ, w->ct);
+    p2g(P, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], w->num, w->den);
+    p2g_epilogue(P, w->ct, w->num, w->den, w->vel, w->saved);
+    divergence(P, w->vel, w->ct, w->occ, w->div);
+    FN(pressure_solve)(P, w->ct, w->div, pr, &st);
+    gradient_enforce(P, pr, w->ct, w->vel);
+}
+
+static void adj_prefix(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pr,
+                       adj_ws *w) {
+    adj_prefix_frozen(P, n, pos, vel, pr, w, NULL, NULL);
+}
+
+/* Gradient-applicable faces of gradient_enforce. */
+static int adj_grad_applies(const orc_params *P, int a, const uint32_t *ct, uint32_t i, uint32_t j, uint32_t k) {
+    const uint32_t nx = P->nx, ny = P->ny;
+    const uint32_t nax = a == 0 ? P->nx : (a == 1 ? P->ny : P->nz);
+    const uint32_t comp = a == 0 ? i : (a == 1 ? j : k);
+    if (comp == 0 || comp == nax) return 0;
+    const uint32_t hi = i + nx * (j + ny * k);
+    const uint32_t lo = a == 0 ? hi - 1u : (a == 1 ? hi - nx : hi - nx * ny);
+    if (ct[lo] == 0u || ct[hi] == 0u) return 0;
+    return ct[lo] == 1u || ct[hi] == 1u;
+}
+
+/* Face flags of the epilogue (forces: wet; boundaries: solid). */
+static void adj_face_flags(const orc_params *P, int a, const uint32_t *ct, uint32_t i, uint32_t j, uint32_t k,
+                           int *wet, int *solid) {
+    const uint32_t nx = P->nx, ny = P->ny, nz = P->nz;
+    const uint32_t nax = a == 0 ? nx : (a == 1 ? ny : nz);
+    const uint32_t comp = a == 0 ? i : (a == 1 ? j : k);
+    int liquid = 0, lo_solid = comp == 0, hi_solid = comp == nax;
+    if (comp > 0) {
+        uint32_t ci = i, cj = j, ck = k;
+        if (a == 0) ci -= 1; else if (a == 1) cj -= 1; else ck -= 1;
+        const uint32_t t = ct[ci + nx * (cj + ny * ck)];
+        if (t == 1u) liquid = 1;
+        if (t == 0u) lo_solid = 1;
+    }
+    if (comp < nax && ct[i + nx * (j + ny * k)] == 1u) liquid = 1;
+    if (!hi_solid) {
+        uint32_t ci = i < nx - 1 ? i : nx - 1, cj = j < ny - 1 ? j : ny - 1, ck = k < nz - 1 ? k : nz - 1;
+        if (ct[ci + nx * (cj + ny * ck)] == 0u) hi_solid = 1;
+    }
+    *wet = liquid;
+    *solid = lo_solid || hi_solid;
+}
+
+/* Exact solve A z = rhs on the liquid cells (plain CG in R). */
+static void adj_solve(const orc_params *P, const uint32_t *ct, const real *rhs, real *z) {
+    const uint32_t C = P->nx * P->ny * P->nz;
+    real *r = (real *)malloc(sizeof(real) * C), *d = (real *)malloc(sizeof(real) * C);
+    real *q = (real *)malloc(sizeof(real) * C);
+    double rr = 0.0;
+    for (uint32_t c = 0; c < C; ++c) {
+        z[c] = (real)0.0;
+        r[c] = ct[c] == 1u ? rhs[c] : (real)0.0;
+        d[c] = r[c];
+        rr += (double)r[c] * (double)r[c];
+    }
+    const double rr0 = rr;
+    const double tol = sizeof(real) == 8 ? 1e-13 : 1e-6;
+    for (int it = 0; it < 20000 && rr > tol * tol * rr0 && rr > 0.0; ++it) {
+        apply_poisson(P, ct, d, q);
+        double dq = 0.0;
+        for (uint32_t c = 0; c < C; ++c) dq += (double)d[c] * (double)q[c];
+        const real alpha = (real)(rr / dq);
+        double rn = 0.0;
+        for (uint32_t c = 0; c < C; ++c) {
+            z[c] += alpha * d[c];
+            r[c] -= alpha * q[c];
+            rn += (double)r[c] * (double)r[c];
+        }
+        const real beta = (real)(rn / rr);
+        for (uint32_t c = 0; c < C; ++c) d[c] = r[c] + beta * d[c];
+        rr = rn;
+    }
+    free(r); free(d); free(q);
+}
+
+/* One substep's adjoint.  pos/vel: the sorted particles before G2P; w: the
+ * forward's intermediates; gx/gv: in, dL/d(output) (sorted order); out,
+ * dL/d(input) (order before the sort).  grav accumulates dL/d(gravity). */
+static void adj_step(const orc_params *P, const adj_tape *T, real *const pos[3], real *const vel[3],
+                     const adj_ws *w, real *const gx[3], real *const gv[3], double grav[3]) {
+    const uint32_t n = T->n, C = T->C, nx = P->nx, ny = P->ny;
+    const real h = (real)P->h, dt = (real)P->dt, alpha = (real)P->flip_blend, inv_h = RINVH(P);
+    const real cfl = RMAX((real)P->cfl, (real)1.0), maxs = cfl * h / dt, hband = h * (real)1.05f;
+    const real ext[3] = {(real)(float)P->nx * h, (real)(float)P->ny * h, (real)(float)P->nz * h};
+    real *vbar[3], *sbar[3], *nbar[3], *dbar[3], *sgx[3], *sgv[3];
+    for (int a = 0; a < 3; ++a) {
+        vbar[a] = (real *)calloc(T->fc[a], sizeof(real));
+        sbar[a] = (real *)calloc(T->fc[a], sizeof(real));
+        nbar[a] = (real *)calloc(T->fc[a], sizeof(real));
+        dbar[a] = (real *)calloc(T->fc[a], sizeof(real));
+        sgx[a] = (real *)calloc(n ? n : 1, sizeof(real));
+        sgv[a] = (real *)calloc(n ? n : 1, sizeof(real));
+    }
+    /* G2P, blend, CFL, advection, clamp. */
+#pragma omp parallel for schedule(static)
+    for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+        const uint32_t p = (uint32_t)pp;
+        const real Pp[3] = {pos[0][p], pos[1][p], pos[2][p]};
+        real pic[3], sv[3], nv0[3], nv1[3], nv1b[3], nv0b[3], xb[3];
+        for (int a = 0; a < 3; ++a) sample_axis(P, a, Pp, w->vel[a], w->saved[a], 1, &pic[a], &sv[a]);
+        for (int a = 0; a < 3; ++a) nv0[a] = pic[a] * ((real)1.0 - alpha) + (vel[a][p] + (pic[a] - sv[a])) * alpha;
+        const real spd = RSQRT(nv0[0] * nv0[0] + nv0[1] * nv0[1] + nv0[2] * nv0[2]);
+        const int capped = spd > maxs && spd > (real)0.0;
+        const real s = capped ? maxs / spd : (real)1.0;
+        for (int a = 0; a < 3; ++a) {
+            nv1[a] = capped ? nv0[a] * s : nv0[a];
+            const real lo = (real)P->origin[a] + hband, hi = (real)P->origin[a] + ext[a] - hband;
+            const real np = Pp[a] + nv1[a] * dt;
+            const int clamped = np < lo || np > hi;
+            nv1b[a] = clamped ? (real)0.0 : gv[a][p] + gx[a][p] * dt;
+            xb[a] = clamped ? (real)0.0 : gx[a][p];
+        }
+        if (capped) {
+            const real nn[3] = {nv0[0] / spd, nv0[1] / spd, nv0[2] / spd};
+            const real dn = nn[0] * nv1b[0] + nn[1] * nv1b[1] + nn[2] * nv1b[2];
+            for (int a = 0; a < 3; ++a) nv0b[a] = s * (nv1b[a] - nn[a] * dn);
+        } else {
+            for (int a = 0; a < 3; ++a) nv0b[a] = nv1b[a];
+        }
+        for (int a = 0; a < 3; ++a) {
+            const real bf = nv0b[a], bs = -alpha * nv0b[a];
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            real g[3];
+            for (int c = 0; c < 3; ++c) g[c] = (Pp[c] - (real)P->origin[c]) * inv_h + k_off[a][c];
+            const int64_t bx = sat_i32(RFLOOR(g[0])), by = sat_i32(RFLOOR(g[1])), bz = sat_i32(RFLOOR(g[2]));
+            const real t3[3] = {g[0] - (real)bx, g[1] - (real)by, g[2] - (real)bz};
+            real W = (real)0.0, dsum[3] = {0, 0, 0};
+            for (int pass = 0; pass < 2; ++pass)
+                for (int dk = 0; dk < 2; ++dk)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int di = 0; di < 2; ++di) {
+                            const int64_t ii = bx + di, jj = by + dj, kk = bz + dk;
+                            if (ii < 0 || ii >= (int64_t)dim[0] || jj < 0 || jj >= (int64_t)dim[1] || kk < 0 ||
+                                kk >= (int64_t)dim[2])
+                                continue;
+                            if (touches_static_solid(P, a, ii, jj, kk)) continue;
+                            const real wx = di ? t3[0] : (real)1.0 - t3[0], wy = dj ? t3[1] : (real)1.0 - t3[1];
+                            const real wz = dk ? t3[2] : (real)1.0 - t3[2];
+                            if (pass == 0) {
+                                W += wx * wy * wz;
+                                continue;
+                            }
+                            if (!(W > (real)0.0)) continue;
+                            const uint64_t idx = (uint64_t)ii + dim[0] * ((uint64_t)jj + dim[1] * (uint64_t)kk);
+                            const real wn = wx * wy * wz / W;
+#pragma omp atomic
+                            vbar[a][idx] += bf * wn;
+#pragma omp atomic
+                            sbar[a][idx] += bs * wn;
+                            const real c = bf * (w->vel[a][idx] - pic[a]) + bs * (w->saved[a][idx] - sv[a]);
+                            const real sx = di ? (real)1.0 : (real)-1.0, sy = dj ? (real)1.0 : (real)-1.0;
+                            const real sz = dk ? (real)1.0 : (real)-1.0;
+                            dsum[0] += sx * wy * wz * c;
+                            dsum[1] += wx * sy * wz * c;
+                            dsum[2] += wx * wy * sz * c;
+                        }
+            if (W > (real)0.0)
+                for (int c = 0; c < 3; ++c) xb[c] += dsum[c] / W * inv_h;
+        }
+        for (int a = 0; a < 3; ++a) {
+            sgx[a][p] = xb[a];
+            sgv[a][p] = alpha * nv0b[a];
+        }
+    }
+    /* Gradient: p_bar of liquid cells, then the exact adjoint solve. */
+    real *pbar = (real *)calloc(C, sizeof(real)), *z = (real *)calloc(C, sizeof(real));
+    const real coeff = (dt / (real)P->rho) / h;
+    for (uint32_t c = 0; c < C; ++c) {
+        if (w->ct[c] != 1u) continue;
+        const uint32_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        real acc = (real)0.0;
+        for (int a = 0; a < 3; ++a) {
+            uint32_t d[3];
+            face_dims(P, a, d);
+            const uint32_t ni = i + (a == 0), nj = j + (a == 1), nk = k + (a == 2);
+            if (adj_grad_applies(P, a, w->ct, i, j, k)) acc -= coeff * vbar[a][i + d[0] * (j + d[1] * k)];
+            if (adj_grad_applies(P, a, w->ct, ni, nj, nk)) acc += coeff * vbar[a][ni + d[0] * (nj + d[1] * nk)];
+        }
+        pbar[c] = acc;
+    }
+    adj_solve(P, w->ct, pbar, z);
+    /* Divergence, epilogue (forces, enforce, save), normalize. */
+    const real bscale = (real)P->rho * h * h / dt;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t d[3];
+        face_dims(P, a, d);
+        const uint32_t nax = a == 0 ? P->nx : (a == 1 ? P->ny : P->nz);
+        double gsum = 0.0;
+        for (uint32_t f = 0; f < T->fc[a]; ++f) {
+            const uint32_t i = f % d[0], j = (f / d[0]) % d[1], k = f / (d[0] * d[1]);
+            int wet, solid;
+            adj_face_flags(P, a, w->ct, i, j, k, &wet, &solid);
+            real ve = solid ? (real)0.0 : vbar[a][f];
+            const uint32_t comp = a == 0 ? i : (a == 1 ? j : k);
+            if (comp > 0) {
+                const uint32_t lo = a == 0 ? (i - 1u) + nx * (j + ny * k)
+                                           : (a == 1 ? i + nx * ((j - 1u) + ny * k) : i + nx * (j + ny * (k - 1u)));
+                if (w->ct[lo] == 1u) ve += inv_h * (-bscale * z[lo]);
+            }
+            if (comp < nax) {
+                const uint32_t hi = i + nx * (j + ny * k);
+                if (w->ct[hi] == 1u) ve -= inv_h * (-bscale * z[hi]);
+            }
+            if (!solid && wet) gsum += (double)ve * (double)dt;
+            const real vn = (solid ? (real)0.0 : ve) + sbar[a][f];
+            const NUMT dd = (NUMT)w->den[a][f];
+            if (dd > 0) {
+                const real den_r = (real)dd / (real)P->fixed_scale;
+                nbar[a][f] = vn / den_r;
+                dbar[a][f] = -vn * w->saved[a][f] / den_r;
+            }
+        }
+        grav[a] += gsum;
+    }
+    /* P2G. */
+#pragma omp parallel for schedule(static)
+    for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+        const uint32_t p = (uint32_t)pp;
+        const real q3[3] = {(pos[0][p] - (real)P->origin[0]) * inv_h, (pos[1][p] - (real)P->origin[1]) * inv_h,
+                            (pos[2][p] - (real)P->origin[2]) * inv_h};
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            const real g[3] = {q3[0] + k_off[a][0], q3[1] + k_off[a][1], q3[2] + k_off[a][2]};
+            const int64_t bx = sat_i32(RFLOOR(g[0])), by = sat_i32(RFLOOR(g[1])), bz = sat_i32(RFLOOR(g[2]));
+            const real t3[3] = {g[0] - (real)bx, g[1] - (real)by, g[2] - (real)bz};
+            for (int dk = 0; dk < 2; ++dk)
+                for (int dj = 0; dj < 2; ++dj)
+                    for (int di = 0; di < 2; ++di) {
+                        const int64_t ii = bx + di, jj = by + dj, kk = bz + dk;
+                        if (ii < 0 || ii >= (int64_t)dim[0] || jj < 0 || jj >= (int64_t)dim[1] || kk < 0 ||
+                            kk >= (int64_t)dim[2])
+                            continue;
+                        const real wx = di ? t3[0] : (real)1.0 - t3[0], wy = dj ? t3[1] : (real)1.0 - t3[1];
+                        const real wz = dk ? t3[2] : (real)1.0 - t3[2];
+                        if (wx * wy * wz <= (real)0.0) continue;
+                        const uint64_t idx = (uint64_t)ii + dim[0] * ((uint64_t)jj + dim[1] * (uint64_t)kk);
+                        const real nb = nbar[a][idx], db = dbar[a][idx];
+                        sgv[a][p] += wx * wy * wz * nb;
+                        const real c = vel[a][p] * nb + db;
+                        const real sx = di ? (real)1.0 : (real)-1.0, sy = dj ? (real)1.0 : (real)-1.0;
+                        const real sz = dk ? (real)1.0 : (real)-1.0;
+                        sgx[0][p] += sx * wy * wz * c * inv_h;
+                        sgx[1][p] += wx * sy * wz * c * inv_h;
+                        sgx[2][p] += wx * wy * sz * c * inv_h;
+                    }
+        }
+    }
+    /* Undo the sort. */
+    for (int a = 0; a < 3; ++a)
+        for (uint32_t d = 0; d < n; ++d) {
+            gx[a][w->perm[d]] = sgx[a][d];
+            gv[a][w->perm[d]] = sgv[a][d];
+        }
+    for (int a = 0; a < 3; ++a) {
+        free(vbar[a]); free(sbar[a]); free(nbar[a]); free(dbar[a]); free(sgx[a]); free(sgv[a]);
+    }
+    free(pbar); free(z);
+}
+
+void FN(adj_free)(void *tape);
+
+static adj_tape *adj_tape_new(const orc_params *P, uint32_t n, uint32_t steps, uint32_t every) {
+    adj_tape *T = (adj_tape *)calloc(1, sizeof(adj_tape));
+    T->P = *P;
+    T->n = n;
+    T->C = P->nx * P->ny * P->nz;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t d[3];
+        face_dims(P, a, d);
+        T->fc[a] = d[0] * d[1] * d[2];
+    }
+    T->steps = steps;
+    T->every = every ? every : 1;
+    T->K = (steps + T->every - 1) / T->every;
+    T->ckpt = (real *)malloc(sizeof(real) * adj_snap_len(T) * (T->K ? T->K : 1));
+    T->itape = (real *)malloc(sizeof(real) * adj_snap_len(T) * T->every);
+    return T;
+}
+
+/* Checkpointed forward: pos/vel/pressure go from the initial to the final
+ * state; returns the tape for FN(adj_backward).  rec_ct / rec_occ
+ * (nullable, steps x C each) receive every substep's cell types and
+ * occupancies. */
+void *FN(adj_forward)(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pressure,
+                      uint32_t steps, uint32_t every, uint32_t *rec_ct, uint32_t *rec_occ) {
+    adj_tape *T = adj_tape_new(P, n, steps, every);
+    adj_ws w;
+    adj_ws_alloc(T, &w);
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (s % T->every == 0) adj_save(T, T->ckpt + (size_t)(s / T->every) * adj_snap_len(T), pos, vel, pressure);
+        adj_prefix(P, n, pos, vel, pressure, &w);
+        if (rec_ct) memcpy(rec_ct + (size_t)s * T->C, w.ct, sizeof(uint32_t) * T->C);
+        if (rec_occ) memcpy(rec_occ + (size_t)s * T->C, w.occ, sizeof(uint32_t) * T->C);
+        g2p(P, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], w.vel, w.saved);
+    }
+    adj_ws_free(&w);
+    return T;
+}
+
+/* The forward with every substep's cell types and occupancies taken from
+ * ct_steps / occ_steps (steps x C each, as FN(adj_forward) records them). */
+void FN(run_frozen)(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pressure,
+                    uint32_t steps, const uint32_t *ct_steps, const uint32_t *occ_steps) {
+    adj_tape *T = adj_tape_new(P, n, 1, 1);
+    adj_ws w;
+    adj_ws_alloc(T, &w);
+    for (uint32_t s = 0; s < steps; ++s) {
+        adj_prefix_frozen(P, n, pos, vel, pressure, &w, ct_steps + (size_t)s * T->C, occ_steps + (size_t)s * T->C);
+        g2p(P, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], w.vel, w.saved);
+    }
+    adj_ws_free(&w);
+    FN(adj_free)(T);
+}
+
+/* A tape whose checkpoints are the given input states of every substep
+ * (states: steps x (pos xyz, vel xyz, pressure), each particle block in that
+ * substep's input order), so FN(adj_backward) runs the reverse pass along a
+ * trajectory computed elsewhere. */
+void *FN(adj_tape_from_states)(const orc_params *P, uint32_t n, uint32_t steps, const real *states) {
+    adj_tape *T = adj_tape_new(P, n, steps, 1);
+    memcpy(T->ckpt, states, sizeof(real) * adj_snap_len(T) * steps);
+    return T;
+}
+
+/* Reverse pass: gpos/gvel in, dL/d(final state) in its order; out,
+ * dL/d(initial state) in the initial order.  grav[3] receives dL/d(gravity). */
+int FN(adj_backward)(void *tape, real *const gpos[3], real *const gvel[3], double grav[3]) {
+    adj_tape *T = (adj_tape *)tape;
+    const orc_params *P = &T->P;
+    const uint32_t n = T->n;
+    real *pos[3], *vel[3], *pr = (real *)malloc(sizeof(real) * T->C);
+    for (int a = 0; a < 3; ++a) {
+        pos[a] = (real *)malloc(sizeof(real) * (n ? n : 1));
+        vel[a] = (real *)malloc(sizeof(real) * (n ? n : 1));
+    }
+    adj_ws w;
+    adj_ws_alloc(T, &w);
+    grav[0] = grav[1] = grav[2] = 0.0;
+    for (int seg = (int)T->K - 1; seg >= 0; --seg) {
+        const uint32_t s0 = (uint32_t)seg * T->every;
+        const uint32_t s1 = s0 + T->every < T->steps ? s0 + T->every : T->steps;
+        adj_load(T, T->ckpt + (size_t)seg * adj_snap_len(T), pos, vel, pr);
+        for (uint32_t s = s0; s < s1; ++s) {
+            adj_save(T, T->itape + (size_t)(s - s0) * adj_snap_len(T), pos, vel, pr);
+            if (s + 1 < s1) {
+                adj_prefix(P, n, pos, vel, pr, &w);
+                g2p(P, n, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], w.vel, w.saved);
+            }
+        }
+        for (int s = (int)s1 - 1; s >= (int)s0; --s) {
+            adj_load(T, T->itape + (size_t)((uint32_t)s - s0) * adj_snap_len(T), pos, vel, pr);
+            adj_prefix(P, n, pos, vel, pr, &w);
+            adj_step(P, T, pos, vel, &w, gpos, gvel, grav);
+        }
+    }
+    adj_ws_free(&w);
+    for (int a = 0; a < 3; ++a) {
+        free(pos[a]); free(vel[a]);
+    }
+    free(pr);
+    return 0;
+}
+
+void FN(adj_free)(void *tape) {
+    adj_tape *T = (adj_tape *)tape;
+    if (!T) return;
+    free(T->ckpt);
+    free(T->itape);
+    free(T);
+}

@@ -732,3 +732,5 @@ void FN(jacobi_solve)(const orc_params *P, const uint32_t *ct, const real *div, 
     memcpy(pout, a, sizeof(real) * C);
     free(a); free(b);
 }
+
+#include "flip_oracle_adjoint.h"

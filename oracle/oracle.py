@@ -64,9 +64,7 @@ _lib = None
 
 def build(force: bool = False) -> str:
     """Compile liboracle.so with the Makefile beside this file."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
-                       stdout=subprocess.DEVNULL)
+    subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True, stdout=subprocess.DEVNULL)
     return LIB_PATH
 
 
@@ -320,3 +318,90 @@ def num_threads() -> int:
 
 def set_num_threads(n: int) -> None:
     lib().orc_set_num_threads(C.c_int(n))
+
+
+class AdjointTape:
+    """Checkpointed forward of the restated step (flip_oracle_adjoint.h) and
+    its reverse pass.  After construction `pos`, `vel`, `pressure` hold the
+    final state; `backward(gpos, gvel)` returns dL/d(initial pos), dL/d(initial
+    vel) and dL/d(gravity) for dL/d(final pos, vel) given in the final order."""
+
+    def __init__(self, P, pos, vel, pressure=None, steps=1, every=10, f64=True, record=False):
+        self.real = np.float64 if f64 else np.float32
+        self.sfx = "f64" if f64 else "f32"
+        n = len(pos)
+        C_ = P.nx * P.ny * P.nz
+        self._cols_p = [np.array(pos[:, a], dtype=self.real, copy=True) for a in range(3)]
+        self._cols_v = [np.array(vel[:, a], dtype=self.real, copy=True) for a in range(3)]
+        self.pressure = np.zeros(C_, self.real) if pressure is None else np.array(pressure, self.real, copy=True)
+        # Every substep's cell types and occupancies (the decisions the
+        # reverse pass holds fixed), for run_frozen.
+        self.ct_steps = np.zeros((steps, C_), np.uint32) if record else None
+        self.occ_steps = np.zeros((steps, C_), np.uint32) if record else None
+        L = lib()
+        fwd = getattr(L, f"orc_adj_forward_{self.sfx}")
+        fwd.restype = C.c_void_p
+        vp = C.c_void_p * 3
+        self._tape = fwd(C.byref(P), C.c_uint32(n), vp(*[ptr(c) for c in self._cols_p]),
+                         vp(*[ptr(c) for c in self._cols_v]), ptr(self.pressure), C.c_uint32(steps),
+                         C.c_uint32(every), ptr(self.ct_steps), ptr(self.occ_steps))
+        self.n = n
+        self.pos = np.stack(self._cols_p, 1)
+        self.vel = np.stack(self._cols_v, 1)
+
+    @classmethod
+    def from_states(cls, P, states, f64=True):
+        """A tape whose substeps start from the given states: a list of
+        (pos [N,3], vel [N,3], pressure [C]) per substep, each in that
+        substep's input order (a trajectory computed elsewhere, e.g. on the
+        GPU).  `backward` then runs the reverse pass along it."""
+        self = cls.__new__(cls)
+        self.real = np.float64 if f64 else np.float32
+        self.sfx = "f64" if f64 else "f32"
+        self.n = len(states[0][0])
+        flat = np.concatenate([np.concatenate([np.asarray(p, self.real).T.ravel(), np.asarray(v, self.real).T.ravel(),
+                                               np.asarray(pr, self.real).ravel()]) for p, v, pr in states])
+        fn = getattr(lib(), f"orc_adj_tape_from_states_{self.sfx}")
+        fn.restype = C.c_void_p
+        self._tape = fn(C.byref(P), C.c_uint32(self.n), C.c_uint32(len(states)), ptr(flat))
+        self.pos = self.vel = self.pressure = None
+        self.ct_steps = self.occ_steps = None
+        return self
+
+    def backward(self, gpos=None, gvel=None):
+        n = self.n
+        gp = [np.zeros(n, self.real) if gpos is None else np.array(gpos[:, a], self.real, copy=True)
+              for a in range(3)]
+        gv = [np.zeros(n, self.real) if gvel is None else np.array(gvel[:, a], self.real, copy=True)
+              for a in range(3)]
+        grav = np.zeros(3, np.float64)
+        vp = C.c_void_p * 3
+        getattr(lib(), f"orc_adj_backward_{self.sfx}")(C.c_void_p(self._tape), vp(*[ptr(c) for c in gp]),
+                                                      vp(*[ptr(c) for c in gv]), ptr(grav))
+        return np.stack(gp, 1), np.stack(gv, 1), grav
+
+    def close(self):
+        if getattr(self, "_tape", None):
+            getattr(lib(), f"orc_adj_free_{self.sfx}")(C.c_void_p(self._tape))
+            self._tape = None
+
+    __del__ = close
+
+
+def run_frozen(P, pos, vel, ct_steps, occ_steps, pressure=None, f64=True):
+    """The forward over len(ct_steps) substeps with every substep's cell
+    types and occupancies taken from ct_steps / occ_steps (as AdjointTape
+    records them): the function the reverse pass differentiates.  Returns
+    the final (pos, vel, pressure)."""
+    real = np.float64 if f64 else np.float32
+    sfx = "f64" if f64 else "f32"
+    cols_p = [np.array(pos[:, a], dtype=real, copy=True) for a in range(3)]
+    cols_v = [np.array(vel[:, a], dtype=real, copy=True) for a in range(3)]
+    pr = np.zeros(P.nx * P.ny * P.nz, real) if pressure is None else np.array(pressure, real, copy=True)
+    ct = np.ascontiguousarray(ct_steps, np.uint32)
+    occ = np.ascontiguousarray(occ_steps, np.uint32)
+    vp = C.c_void_p * 3
+    getattr(lib(), f"orc_run_frozen_{sfx}")(C.byref(P), C.c_uint32(len(pos)), vp(*[ptr(c) for c in cols_p]),
+                                            vp(*[ptr(c) for c in cols_v]), ptr(pr), C.c_uint32(len(ct)), ptr(ct),
+                                            ptr(occ))
+    return np.stack(cols_p, 1), np.stack(cols_v, 1), pr
