@@ -442,7 +442,8 @@ def main():
         traffic = None
 
     line = {
-        "metric": "FLIP particle-steps/s @128^3: mixed vs FP32 CG at 1e-5 residual; CG %HBM peak",
+        "metric": ("FLIP particle-steps/s @128^3: mixed vs FP32 CG at 1e-5 residual; CG %HBM peak"
+                   if sc["name"] == "high_density_128" else f"FLIP particle-steps/s ({sc['name']})"),
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -488,7 +489,7 @@ def main():
         cdt, res, cores = cpu_baseline(sc, args.cpu_steps)
         line["cpu_baseline"] = {
             "value": N * args.cpu_steps / cdt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{args.cpu_steps} substeps of the 128^3 workload from its initial state, plain FP32 "
+            "sample": f"{args.cpu_steps} substeps of the {sc['name']} workload from its initial state, plain FP32 "
                       f"restatement (oracle/), OpenMP {cores} threads, {cdt:.1f} s",
         }
     print(json.dumps(line), flush=True)

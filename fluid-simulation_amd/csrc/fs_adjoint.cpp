@@ -25,8 +25,12 @@ using namespace fs;
 
 namespace {
 
-constexpr float kAdjTol = 1e-6f;       // relative residual of the adjoint solve
-constexpr uint32_t kAdjIters = 1000;   // its iteration cap
+// The adjoint solve runs to the forward's relative tolerance (1e-6 when the
+// forward runs a fixed iteration count), capped at the larger of the
+// forward's cap and kAdjIters.  Its error enters the gradient at that
+// relative level.
+constexpr float kAdjTolDefault = 1e-6f;
+constexpr uint32_t kAdjIters = 400;
 
 int alloc_snapshot(fs_tank* t, fs_tank::Snapshot& s) {
     for (int a = 0; a < 3; ++a) {
@@ -107,8 +111,8 @@ int adjoint_substep(fs_tank* t) {
     FS_HIP(launch_adj_pbar(P, t->ct, faces, t->adj_pbar, s));
     FS_HIP(launch_adj_cg_init(P, t->cg.nmask, t->adj_pbar, t->adj_rhs, t->adj_z, t->r, t->d0, s));
     StepParams Pa = P;
-    Pa.tol = kAdjTol;
-    Pa.tol_sq = kAdjTol * kAdjTol;
+    Pa.tol = P.tol > 0.0f ? P.tol : kAdjTolDefault;
+    Pa.tol_sq = Pa.tol * Pa.tol;
     Pa.iters = kAdjIters > P.iters ? kAdjIters : P.iters;
     FS_HIP(launch_fill_u32(t->cg.bar_count, 0u, 1, s));
     FS_HIP(launch_fill_u32(t->cg.group_count, 0u, 8 * 32, s));

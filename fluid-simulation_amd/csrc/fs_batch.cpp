@@ -415,7 +415,7 @@ FS_API int fs_batch_create(const fs_params* params, const float* const* pos_xyz,
         if (const char* env = std::getenv("FLUIDSTEP_BATCH_GRAPH")) bt->use_graph = std::atoi(env) != 0;
         // Work tables.
         std::vector<WorkItem> part, cell, face, brick[3];
-        static const int bx[3] = {16, 8, 32}, by[3] = {8, 8, 4}, bz[3] = {8, 8, 4};
+        static const int bx[3] = {8, 16, 32}, by[3] = {8, 8, 4}, bz[3] = {8, 8, 4};
         for (uint32_t t = 0; t < ntanks; ++t) {
             const StepParams& P = bt->tanks[t].P;
             for (uint32_t s0 = 0; s0 < P.N; s0 += kBatchBlock) part.push_back({t, s0});

@@ -337,7 +337,7 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
     if (sort_now) {
         FS_HIP(launch_p2g_gather(P, live, t->side[t->cur ^ 1], t->bf16, t->occ, t->cell_offset, t->ct, t->vel,
-                                 t->saved, t->counters + 1, t->mis_list, s));
+                                 t->saved, t->counters + 1, t->mis_list, s, t->gather_brick));
         // Fallback when more particles' P2G cells differ from their sort
         // cells than the gather walks.
         for (int a = 0; a < 3; ++a) {
@@ -590,6 +590,10 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         }
         if (const char* env = std::getenv("FLUIDSTEP_CG_MODE")) t->cg_mode = std::strcmp(env, "persistent") == 0 ? 0 : 1;
         if (const char* env = std::getenv("FLUIDSTEP_CG_GRAPH")) t->cg_use_graph = std::atoi(env) != 0;
+        if (const char* env = std::getenv("FLUIDSTEP_GATHER_BRICK")) {
+            const int v = std::atoi(env);
+            if (v >= 0 && v <= 2) t->gather_brick = v;
+        }
         if (std::getenv("FLUIDSTEP_VERBOSE"))
             std::fprintf(stderr, "fluidstep: CG %s, persistent grid %d (co-resident max %d), multi grid %d, graph %d\n",
                          t->cg_mode == 0 ? "persistent" : "multi-kernel", t->cg_blocks, max_blocks,

@@ -73,7 +73,7 @@ hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3]
 hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const uint8_t* ct,
                                    float* const vel[3], hipStream_t s);
 // Sorted-segment P2G gather per brick of cells, with the face epilogue fused.
-// brick_variant: 0 = 16x8x8, 1 = 8x8x8, 2 = 32x4x4 cells per workgroup.
+// brick_variant: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells per workgroup.
 // src is the pre-sort side, which the listed misfit indices refer to; the
 // gather does nothing when *mis_count exceeds kMisfitCap (fs_gather.h).
 hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,

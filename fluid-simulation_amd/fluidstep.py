@@ -112,7 +112,7 @@ class Tuning(C.Structure):
         ("gather_brick", C.c_uint32), ("sort_tile", C.c_uint32), ("cg_unroll", C.c_uint32),
         ("cg_blocks", C.c_uint32), ("ms_gather", C.c_double), ("ms_sort", C.c_double), ("ms_cg", C.c_double),
     ]
-    BRICKS = ("16x8x8", "8x8x8", "32x4x4")
+    BRICKS = ("8x8x8", "16x8x8", "32x4x4")
     SORT_TILES = (4096, 2048, 8192)
     CG_UNROLL = (4, 2, 1)
 

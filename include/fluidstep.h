@@ -197,7 +197,7 @@ typedef struct fs_batch fs_batch;
 
 /* Tile shapes the autotuner picks from (each an index into a fixed menu). */
 typedef struct fs_batch_tuning {
-    uint32_t gather_brick; /* P2G gather brick: 0 = 16x8x8, 1 = 8x8x8, 2 = 32x4x4 cells */
+    uint32_t gather_brick; /* P2G gather brick: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells */
     uint32_t sort_tile;    /* radix-sort keys per workgroup: 0 = 4096, 1 = 2048, 2 = 8192 */
     uint32_t cg_unroll;    /* CG tiles in flight per wave: 0 = 4, 1 = 2, 2 = 1 */
     uint32_t cg_blocks;    /* persistent CG workgroups (0 = default) */
