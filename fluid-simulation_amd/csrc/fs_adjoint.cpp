@@ -114,10 +114,11 @@ int adjoint_substep(fs_tank* t) {
     Pa.tol = P.tol > 0.0f ? P.tol : kAdjTolDefault;
     Pa.tol_sq = Pa.tol * Pa.tol;
     Pa.iters = kAdjIters > P.iters ? kAdjIters : P.iters;
-    FS_HIP(launch_fill_u32(t->cg.bar_count, 0u, 1, s));
-    FS_HIP(launch_fill_u32(t->cg.group_count, 0u, 8 * 32, s));
-    FS_HIP(launch_fill_u32(t->cg.abort_flag, 0u, 1, s));
-    FS_HIP(launch_cg_persistent(Pa, t->ct, t->adj_rhs, t->adj_z, t->r, t->d0, t->d1, t->q, t->cg, t->cg_blocks, s));
+    FS_HIP(launch_cg_reset(t->cg, s));
+    CgVectors av = t->cg_vectors();
+    av.b = t->adj_rhs;
+    av.x = t->adj_z;
+    FS_HIP(launch_cg_persistent(Pa, av, t->cg, t->cg_blocks, s));
     FS_HIP(launch_adj_faces(P, t->ct, t->adj_z, faces, t->adj_grav, s));
     FS_HIP(launch_adj_p2g(P, sorted, faces, part_sorted, s));
     FS_HIP(launch_adj_unsort(P.N, t->perm, part_sorted, part_in, s));

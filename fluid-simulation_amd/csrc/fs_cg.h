@@ -146,6 +146,131 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
     return acc;
 }
 
+// ---- single-reduction form (Chronopoulos & Gear 1989) ----
+// With s = A pd and w = A r kept as vectors, one sweep per iteration does
+//     pd = r + beta pd;  s = w + beta s;  x += alpha pd;  r' = r - alpha s;
+//     w' = A r';  partial sums of r'.r' and w'.r'
+// where alpha and beta come from the previous sweep's two sums.  A r' needs r'
+// at the six neighbours: the sweep recomputes it there from the neighbours'
+// r, w and s with the owner's exact f32 expression, so one grid barrier per
+// iteration suffices.  r, w and s are read at neighbours while their next
+// values are written, so they alternate between two buffers.
+struct CgSweepIo {
+    const float* r_in;   // r_k
+    const float* w_in;   // w_k = A r_k
+    const float* s_in;   // s_{k-1} (unread when first)
+    float* r_out;        // r_{k+1}
+    float* w_out;        // w_{k+1}
+    float* s_out;        // s_k
+    float* pd;           // search direction, own cells only
+    float* x;            // solution, own cells only
+};
+
+// Initial sweep: w_0 = A r_0 over the tiles; returns partial r.r and w.r.
+template <int U, typename TileId>
+__device__ __forceinline__ void cgs_init_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                               const float* __restrict__ r, float* __restrict__ w_out,
+                                               const TileWalk& tw, uint32_t ti, TileId&& tile_id, double& gam,
+                                               double& del) {
+    const uint32_t lane = lane_id(), C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    uint32_t c[U];
+    uint16_t m[U];
+    float rc[U], rn[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+        const bool ok = c[u] < C;
+        rc[u] = ok ? r[c[u]] : 0.0f;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            int32_t nb = (int32_t)c[u] + off[s];
+            nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
+            rn[u][s] = ok ? r[nb] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (c[u] >= C) continue;
+        float wc = 0.0f;
+        if (m[u] & kMaskLiquid) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if ((m[u] >> (3 + s)) & 1u) sum += rn[u][s];
+            wc = (float)(m[u] & 7u) * rc[u] - sum;
+        }
+        w_out[c[u]] = wc;
+        gam += (double)rc[u] * (double)rc[u];
+        del += (double)wc * (double)rc[u];
+    }
+}
+
+// One iteration's sweep over U tiles; accumulates r'.r' and w'.r'.
+template <int U, typename TileId>
+__device__ __forceinline__ void cgs_sweep_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                                const CgSweepIo& io, float alpha, float beta, bool first,
+                                                const TileWalk& tw, uint32_t ti, TileId&& tile_id, double& gam,
+                                                double& del) {
+    const uint32_t lane = lane_id(), C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    const float* __restrict__ r = io.r_in;
+    const float* __restrict__ w = io.w_in;
+    const float* __restrict__ sp = io.s_in;
+    uint32_t c[U];
+    uint16_t m[U];
+    float rc[U], wc[U], sc[U], pc[U], xc[U], rn[U][6], wn[U][6], sn[U][6];
+    // Every load is issued before any result is used (clamped neighbour
+    // indices; the mask selects the terms afterwards).
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+        const bool ok = c[u] < C;
+        rc[u] = ok ? r[c[u]] : 0.0f;
+        wc[u] = ok ? w[c[u]] : 0.0f;
+        sc[u] = (ok && !first) ? sp[c[u]] : 0.0f;
+        pc[u] = (ok && !first) ? io.pd[c[u]] : 0.0f;
+        xc[u] = ok ? io.x[c[u]] : 0.0f;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            int32_t nb = (int32_t)c[u] + off[s];
+            nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
+            rn[u][s] = ok ? r[nb] : 0.0f;
+            wn[u][s] = ok ? w[nb] : 0.0f;
+            sn[u][s] = (ok && !first) ? sp[nb] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (c[u] >= C) continue;
+        const float s_k = first ? wc[u] : wc[u] + beta * sc[u];
+        const float pd_k = first ? rc[u] : rc[u] + beta * pc[u];
+        const float r1 = rc[u] - alpha * s_k;
+        float w1 = 0.0f;
+        if (m[u] & kMaskLiquid) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if ((m[u] >> (3 + s)) & 1u) {
+                    const float s_nb = first ? wn[u][s] : wn[u][s] + beta * sn[u][s];
+                    sum += rn[u][s] - alpha * s_nb;
+                }
+            w1 = (float)(m[u] & 7u) * r1 - sum;
+        }
+        io.pd[c[u]] = pd_k;
+        io.x[c[u]] = xc[u] + alpha * pd_k;
+        io.s_out[c[u]] = s_k;
+        io.r_out[c[u]] = r1;
+        io.w_out[c[u]] = w1;
+        gam += (double)r1 * (double)r1;
+        del += (double)w1 * (double)r1;
+    }
+}
+
 template <int U, typename TileId>
 __device__ __forceinline__ double rr_batch(const StepParams& P, const float* __restrict__ r, const TileWalk& tw,
                                            uint32_t ti, TileId&& tile_id) {

@@ -208,13 +208,23 @@ int check_params(const fs_params& q, uint32_t n, const float* pos_xyz, const flo
     return FS_OK;
 }
 
-// Co-resident grid for a persistent kernel: the occupancy figure is not a
-// residency guarantee (on MI355X a grid of the full figure did not become
-// resident at once and its grid barrier timed out, while half of it did), so
-// half of it, in whole XCD groups (multiple of 8 blocks).
+// Grid of a persistent (cooperative) kernel whose co-resident figure
+// (occupancy x CUs) is max_blocks.  A grid barrier's latency grows with the
+// number of workgroups (on MI355X about 7 us at 256 and 11 us at 512), so a
+// kernel resident at two or more workgroups per CU runs half its figure; one
+// resident at one workgroup per CU runs one per CU.  Whole XCD groups
+// (multiples of 8 blocks) either way.
 int half_resident_blocks(int max_blocks) {
     int b = (max_blocks / 2 / 8) * 8;
     return b > 0 ? b : 1;
+}
+
+int persistent_grid_blocks(int max_blocks, int cus) {
+    if (max_blocks <= cus) {
+        const int b = (max_blocks / 8) * 8;
+        return b > 0 ? b : 1;
+    }
+    return half_resident_blocks(max_blocks);
 }
 
 }  // namespace fs
@@ -358,12 +368,10 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[1], s));
     FS_HIP(launch_div_cginit(P, t->vel, t->ct, t->occ, t->div, t->b, t->p, t->r, t->d0, t->cg, t->scan_ws, s));
-    FS_HIP(launch_fill_u32(t->cg.bar_count, 0u, 1, s));
-    FS_HIP(launch_fill_u32(t->cg.group_count, 0u, 8 * 32, s));
-    FS_HIP(launch_fill_u32(t->cg.abort_flag, 0u, 1, s));
+    FS_HIP(launch_cg_reset(t->cg, s));
     if (t->timing) FS_HIP(hipEventRecord(t->ev[4], s));
     if (t->cg_mode == 0) {
-        FS_HIP(launch_cg_persistent(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_blocks, s));
+        FS_HIP(launch_cg_persistent(P, t->cg_vectors(), t->cg, t->cg_blocks, s));
     } else if (!t->cg_use_graph) {
         FS_HIP(launch_cg_multi(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_multi_blocks, s));
     } else {
@@ -566,23 +574,28 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(t->alloc(t->d0, C));
         FS_HIP(t->alloc(t->d1, C));
         FS_HIP(t->alloc(t->q, C));
+        FS_HIP(t->alloc(t->r1, C));
+        for (int k = 0; k < 2; ++k) {
+            FS_HIP(t->alloc(t->cg_w[k], C));
+            FS_HIP(t->alloc(t->cg_s[k], C));
+        }
         FS_HIP(t->alloc(t->scal, 8));
         FS_HIP(t->alloc(t->partials, (C + 255) / 256));
         FS_HIP(t->alloc(t->counters, 4));
         FS_HIP(t->alloc(t->mis_list, kMisfitCap));
         const size_t ntiles = C / 64 + 1;
+        int dev = 0, cus = 0;
+        FS_HIP(hipGetDevice(&dev));
+        FS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         t->cg_blocks = cg_persistent_max_blocks();
         if (t->cg_blocks <= 0) return fail(FS_ERR_DEVICE, "cannot size the persistent CG grid");
         const int max_blocks = t->cg_blocks;
-        t->cg_blocks = half_resident_blocks(t->cg_blocks);
+        t->cg_blocks = persistent_grid_blocks(max_blocks, cus);
         if (const char* env = std::getenv("FLUIDSTEP_CG_BLOCKS")) {
             int v = std::atoi(env);
             if (v > 0 && v <= max_blocks) t->cg_blocks = v;
         }
         if (t->cg_blocks <= 0) t->cg_blocks = 1;
-        int dev = 0, cus = 0;
-        FS_HIP(hipGetDevice(&dev));
-        FS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         t->cg_multi_blocks = 4 * cus;
         if (const char* env = std::getenv("FLUIDSTEP_CG_MULTI_BLOCKS")) {
             int v = std::atoi(env);
@@ -598,13 +611,13 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
             std::fprintf(stderr, "fluidstep: CG %s, persistent grid %d (co-resident max %d), multi grid %d, graph %d\n",
                          t->cg_mode == 0 ? "persistent" : "multi-kernel", t->cg_blocks, max_blocks,
                          t->cg_multi_blocks, t->cg_use_graph);
-        FS_HIP(t->alloc(t->cg.bar_count, 1));
-        FS_HIP(t->alloc(t->cg.group_count, 8 * 32));
+        uint32_t* sync = nullptr;
+        FS_HIP(t->alloc(sync, cg_sync_words(max_blocks)));
+        cg_bind_sync(t->cg, sync, max_blocks);
         FS_HIP(t->alloc(t->cg.nmask, C));
-        FS_HIP(t->alloc(t->cg.bar_gen, 1));
-        FS_HIP(t->alloc(t->cg.abort_flag, 1));
-        // Two halves: the persistent form alternates them between reductions.
-        FS_HIP(t->alloc(t->cg.partials, 2 * (size_t)std::max(t->cg_blocks, t->cg_multi_blocks)));
+        // Two halves of up to two sums per block: the persistent form alternates
+        // the halves between reductions.
+        FS_HIP(t->alloc(t->cg.partials, 2 * 2 * (size_t)std::max(max_blocks, t->cg_multi_blocks)));
         FS_HIP(t->alloc(t->cg.result, 4));
         FS_HIP(t->alloc(t->cg.tile_flags, ntiles));
         FS_HIP(t->alloc(t->cg.tile_scan, ntiles));
@@ -623,7 +636,7 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(launch_fill_u32(t->cell_offset, 0u, C, t->stream));
         FS_HIP(launch_fill_u32((uint32_t*)t->scal, 0u, 8, t->stream));
         FS_HIP(launch_fill_u32(t->counters, 0u, 4, t->stream));
-        FS_HIP(launch_fill_u32(t->cg.bar_gen, 0u, 1, t->stream));
+        FS_HIP(launch_cg_reset(t->cg, t->stream));
         FS_HIP(launch_fill_u32((uint32_t*)t->cg.state, 0u, sizeof(CgState) / 4, t->stream));
         // cg_scalars[6] = tol_sq (set_pressure_residual_tolerance)
         FS_HIP(hipMemcpyAsync(t->scal + 6, &t->P.tol_sq, sizeof(float), hipMemcpyHostToDevice, t->stream));

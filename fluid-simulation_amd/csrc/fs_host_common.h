@@ -22,6 +22,7 @@ bool all_finite(const float* a, size_t n);
 int check_params(const fs_params& q, uint32_t n, const float* pos_xyz, const float* vel_xyz);
 // The persistent grids' block count from the occupancy figure.
 int half_resident_blocks(int max_blocks);
+int persistent_grid_blocks(int max_blocks, int cus);
 
 }  // namespace fs
 

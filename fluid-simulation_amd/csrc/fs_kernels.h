@@ -111,12 +111,17 @@ struct CgState {
     uint32_t pad;
 };
 struct CgWork {
-    // grid-barrier + reduction scratch
+    // grid-barrier + reduction scratch; every word below up to `slots`
+    // (inclusive) lives in one area of cg_sync_words(max_blocks) u32 words
+    // that launch_cg_reset clears before each solve
+    uint32_t* sync;          // base of the area
+    uint32_t sync_words;
     uint32_t* bar_count;     // [1]
     uint32_t* group_count;   // [8 * 32]: one arrival counter per 128-byte line
     uint32_t* bar_gen;       // [1]
-    uint16_t* nmask;         // [C] stencil mask of liquid cells
     uint32_t* abort_flag;    // [1]
+    uint64_t* slots;         // [2][max_blocks][2] epoch-tagged partials
+    uint16_t* nmask;         // [C] stencil mask of liquid cells
     double* partials;        // [max_blocks]
     double* result;          // [4]
     uint32_t* tile_flags;    // [C/64 + 1]
@@ -131,10 +136,34 @@ hipError_t launch_compact_tiles(uint32_t ntiles, const uint32_t* flags, const ui
 hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uint8_t* ct,
                              const uint32_t* occ, float* div, float* b, float* p, float* r, float* d,
                              const CgWork& w, uint32_t* scan_ws, hipStream_t s);
-hipError_t launch_cg_persistent(const StepParams& P, const uint8_t* ct, const float* b, float* p,
-                                float* r, float* d0, float* d1, float* q, const CgWork& w,
-                                int grid_blocks, hipStream_t s);
+// Vectors of the persistent solve.  On entry r[0] holds r_0 = b - A x_0 and
+// pd holds d_0 = r_0 (k_div_cginit / the adjoint's init write both); x is
+// updated in place.  The single-reduction form uses all of them; the
+// two-phase form uses b, x, r[0] (residual), pd and s[0] (directions) and
+// w[0] (A d).
+struct CgVectors {
+    const float* b;
+    float* x;
+    float* r[2];
+    float* w[2];
+    float* s[2];
+    float* pd;
+};
+
+// Size of the sync area for a grid of up to max_blocks, and the binding of
+// the counters, flags and slots to an allocation of that many words.
+size_t cg_sync_words(int max_blocks);
+void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks);
+// Clears the sync area (one fill kernel); enqueue before every solve.
+hipError_t launch_cg_reset(const CgWork& w, hipStream_t s);
+// The whole solve in one cooperative launch; partials must hold
+// 2 x 2 x grid_blocks doubles.
+hipError_t launch_cg_persistent(const StepParams& P, const CgVectors& v, const CgWork& w, int grid_blocks,
+                                hipStream_t s);
+// Co-resident blocks of the persistent kernel (occupancy x CUs) and its
+// workgroup size.
 int cg_persistent_max_blocks();
+int cg_persistent_threads();
 // Same solve as two kernels per iteration (SpMV+d.q, update+r.r) with
 // last-block-done reductions; P.iters iterations are enqueued (converged
 // iterations return at once).  Suitable for hipGraph capture.

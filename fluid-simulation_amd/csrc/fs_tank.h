@@ -35,11 +35,16 @@ struct fs_tank {
     int32_t *num[3] = {}, *den[3] = {};
     float *vel[3] = {}, *saved[3] = {};
     float *div = nullptr, *b = nullptr, *p = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr, *q = nullptr;
+    float *r1 = nullptr, *cg_w[2] = {}, *cg_s[2] = {};  // single-reduction CG: r, w = A r, s = A pd
     float* scal = nullptr;      // reference CG scalars [8]
     float* partials = nullptr;  // reference CG partials
     uint32_t* counters = nullptr;  // [0] liquid cells, [1] misfits, [2] CG iterations (reference engine)
     uint32_t* mis_list = nullptr;  // indices (pre-sort side) of the first kMisfitCap misfit particles
     CgWork cg{};
+    // Persistent-solve vectors of the forward step (b, p, r|r1, cg_w, cg_s, d0).
+    fs::CgVectors cg_vectors() const {
+        return fs::CgVectors{b, p, {r, r1}, {cg_w[0], cg_w[1]}, {cg_s[0], cg_s[1]}, d0};
+    }
     int cg_blocks = 0;        // persistent form
     int cg_mode = 0;          // 0 persistent kernel, 1 two kernels per iteration
     int cg_multi_blocks = 0;  // multi-kernel form

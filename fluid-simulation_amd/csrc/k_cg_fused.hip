@@ -20,6 +20,8 @@
 //    src/sim/pressure.rs:282,296 do; the stop test rs_new <= tol^2 rs_init is
 //    f64 (src/sim/pressure.rs:262-292).
 //  * the f64 true residual |b - A p|^2 is reduced after the loop.
+#include <cstdlib>
+#include <cstring>
 #include "fs_cell.h"
 #include "fs_cg.h"
 
@@ -27,10 +29,26 @@ namespace fs {
 
 namespace {
 inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
-constexpr int kCgThreads = 512;       // persistent workgroup (8 waves)
 constexpr int kCgMultiThreads = 256;  // multi-kernel workgroup
 constexpr int kLdsTiles = 2048;       // tile ids a persistent block keeps in LDS
+// Sync area layout (u32 words): each counter or flag on its own 128-byte line.
+constexpr size_t kSyncBarCount = 0, kSyncGroups = 32, kSyncAbort = 32 + 8 * 32, kSyncGen = kSyncAbort + 32,
+                 kSyncSlots = kSyncGen + 32;
 }  // namespace
+
+size_t cg_sync_words(int max_blocks) { return kSyncSlots + 8u * (size_t)(max_blocks > 0 ? max_blocks : 1); }
+
+void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks) {
+    w.sync = area;
+    w.sync_words = (uint32_t)cg_sync_words(max_blocks);
+    w.bar_count = area + kSyncBarCount;
+    w.group_count = area + kSyncGroups;
+    w.abort_flag = area + kSyncAbort;
+    w.bar_gen = area + kSyncGen;
+    w.slots = reinterpret_cast<uint64_t*>(area + kSyncSlots);
+}
+
+hipError_t launch_cg_reset(const CgWork& w, hipStream_t s) { return launch_fill_u32(w.sync, 0u, w.sync_words, s); }
 
 __global__ void k_div_cginit(StepParams P, const float* __restrict__ u, const float* __restrict__ v,
                              const float* __restrict__ w, const uint8_t* __restrict__ ct,
@@ -109,22 +127,75 @@ __device__ __forceinline__ bool arrive(const CgWork& w) {
     return true;
 }
 
-// Grid barrier + sum for the persistent kernel.  The last arrival only bumps
-// the generation word; afterwards every block sums the G published partials
-// itself in the same fixed order, so all blocks hold the identical total
-// without another round trip.  Partials alternate between two halves of the
-// array so a fast block's next publication cannot overwrite a value a slow
-// block is still summing.  Returns false on abort.
-template <int THREADS>
-__device__ bool grid_sum(double v, const CgWork& w, double& out) {
-    __shared__ double s_w[THREADS / 64];
-    __shared__ double s_tree[THREADS];
+// Block partials of NV values: wave sums (lane 0's butterfly result), then
+// thread 0 adds the waves in order.  Valid in thread 0 only.
+template <int THREADS, int NV>
+__device__ __forceinline__ void block_sum_nv(const double (&v)[NV], double (&bs)[NV]) {
+    __shared__ double s_w[NV][THREADS / 64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const double t = wave_sum(v[j]);
+        if (lane == 0) s_w[j][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double t = 0.0;
+            for (int i = 0; i < THREADS / 64; ++i) t += s_w[j][i];
+            bs[j] = t;
+        }
+    }
+}
+
+// Total of G published partials in one fixed order: thread i has added
+// partials i, i + THREADS, ... into `loaded`; waves reduce by butterfly
+// (lane 0's result); thread 0 adds the wave results in order.  Every block
+// therefore forms the bit-identical total, returned to all threads.
+template <int THREADS, int NV>
+__device__ __forceinline__ void reduce_loaded(const double (&loaded)[NV], double (&out)[NV]) {
+    __shared__ double s_w[NV][THREADS / 64];
+    __shared__ double s_out[NV];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const double t = wave_sum(loaded[j]);
+        if (lane == 0) s_w[j][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double t = 0.0;
+            for (int i = 0; i < THREADS / 64; ++i) t += s_w[j][i];
+            s_out[j] = t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NV; ++j) out[j] = s_out[j];
+}
+
+// Grid barrier + sum for the persistent kernels.  Orders memory: every
+// block's stores before the barrier are visible to every block after it.
+// The last arrival only bumps the generation word; afterwards every block
+// sums the G published partials itself in the same fixed order, so all
+// blocks hold the identical total without another round trip.  Partials
+// alternate between two halves of the array so a fast block's next
+// publication cannot overwrite a value a slow block is still summing.
+// Returns false on abort.
+template <int THREADS, int NV>
+__device__ bool grid_sum_nv(const double (&v)[NV], const CgWork& w, double (&out)[NV]) {
     __shared__ int s_last, s_abort;
     __shared__ uint32_t s_gen;
-    const double bs = block_sum<THREADS>(v, s_w);
+    double bs[NV];
+    block_sum_nv<THREADS, NV>(v, bs);
+    const uint32_t G = gridDim.x;
     if (threadIdx.x == 0) {
         const uint32_t g0 = ld_relaxed(w.bar_gen);
-        w.partials[(size_t)(g0 & 1u) * gridDim.x + blockIdx.x] = bs;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) w.partials[((size_t)(g0 & 1u) * G + blockIdx.x) * NV + j] = bs[j];
         s_gen = g0;
         s_abort = 0;
         s_last = arrive(w) ? 1 : 0;
@@ -151,7 +222,76 @@ __device__ bool grid_sum(double v, const CgWork& w, double& out) {
     }
     __syncthreads();
     if (s_abort) return false;
-    out = sum_partials<THREADS>(w.partials + (size_t)(s_gen & 1u) * gridDim.x, gridDim.x, s_tree);
+    const double* base = w.partials + (size_t)(s_gen & 1u) * G * NV;
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+    for (uint32_t i = threadIdx.x; i < G; i += THREADS)
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            acc[j] += __hip_atomic_load(&base[(size_t)i * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    reduce_loaded<THREADS, NV>(acc, out);
+    return true;
+}
+
+template <int THREADS>
+__device__ __forceinline__ bool grid_sum(double v, const CgWork& w, double& out) {
+    const double in[1] = {v};
+    double o[1];
+    if (!grid_sum_nv<THREADS, 1>(in, w, o)) return false;
+    out = o[0];
+    return true;
+}
+
+// Reduction without counters or fences, for a total whose consumers read no
+// data another block wrote since the previous grid_sum.  Every block stores
+// its partial as two 64-bit words, each tagged with the epoch in its upper
+// half (high and low halves of the f64 in the lower), in its own slot, and
+// reads all G slots until each carries the epoch; the total is then formed
+// in grid_sum's order.  Tagged words are written and read with agent-scope
+// relaxed atomics, which go to memory around the per-XCD L2, so no
+// write-back or invalidate is issued.  Slots alternate between two halves by
+// epoch parity: a block publishes epoch e+1 only after reading every epoch-e
+// slot, and epoch e+2 only after every block has published e+1, i.e.
+// finished reading epoch e.  The slots start cleared (tag 0) and epochs
+// start at 1.  Returns false on abort.
+template <int THREADS>
+__device__ bool grid_exchange(double v, const CgWork& w, uint32_t epoch, double& out) {
+    __shared__ int s_abort;
+    const uint32_t G = gridDim.x;
+    uint64_t* half = w.slots + (size_t)(epoch & 1u) * G * 2u;
+    const uint64_t tag = (uint64_t)epoch << 32;
+    const double in[1] = {v};
+    double bs[1];
+    block_sum_nv<THREADS, 1>(in, bs);
+    if (threadIdx.x == 0) {
+        s_abort = 0;
+        const uint64_t bits = (uint64_t)__double_as_longlong(bs[0]);
+        __hip_atomic_store(half + 2u * blockIdx.x, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(half + 2u * blockIdx.x + 1u, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    double acc[1] = {0.0};
+    for (uint32_t i = threadIdx.x; i < G; i += THREADS) {
+        const long long t0 = wall_clock64();
+        uint64_t hi, lo;
+        for (uint32_t spin = 0;; ++spin) {
+            hi = __hip_atomic_load(half + 2u * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lo = __hip_atomic_load(half + 2u * i + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(hi >> 32) == epoch && (uint32_t)(lo >> 32) == epoch) break;
+            if ((spin & 31u) == 31u && (ld_relaxed(w.abort_flag) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+                __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_abort = 1;
+                break;
+            }
+        }
+        acc[0] += __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+    }
+    double o[1];
+    reduce_loaded<THREADS, 1>(acc, o);
+    if (s_abort) return false;
+    out = o[0];
     return true;
 }
 
@@ -174,7 +314,10 @@ __device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
 }
 
 // ---- persistent form ----
-__global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, const uint8_t* __restrict__ ct,
+// THREADS: workgroup size; EXCH: the reduction after phase A (dn.q, which
+// phase B consumes with data only its own lanes wrote) uses grid_exchange.
+template <int THREADS, bool EXCH>
+__global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const uint8_t* __restrict__ ct,
                                                                const float* __restrict__ b, float* __restrict__ p,
                                                                float* __restrict__ r, float* __restrict__ d0,
                                                                float* __restrict__ d1, float* __restrict__ q,
@@ -184,13 +327,13 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
     const uint32_t G = gridDim.x;
     const uint32_t nt = *w.n_tiles;
     const uint32_t lb = xcd_remap(blockIdx.x, G);
-    const TileWalk tw = tile_walk(nt, lb, G, kCgThreads / 64);
+    const TileWalk tw = tile_walk(nt, lb, G, THREADS / 64);
     // This block's slice of the tile list, kept in LDS when it fits.
     const uint32_t s0 = (uint32_t)(((uint64_t)nt * lb) / G);
     const uint32_t s1 = (uint32_t)(((uint64_t)nt * (lb + 1u)) / G);
     const bool in_lds = s1 - s0 <= (uint32_t)kLdsTiles;
     if (in_lds)
-        for (uint32_t i = s0 + threadIdx.x; i < s1; i += kCgThreads) s_tiles[i - s0] = w.tiles[i];
+        for (uint32_t i = s0 + threadIdx.x; i < s1; i += THREADS) s_tiles[i - s0] = w.tiles[i];
     __syncthreads();
     const uint32_t* gt = w.tiles;
     auto tile_id = [&](uint32_t t) -> uint32_t { return in_lds ? s_tiles[t - s0] : gt[t]; };
@@ -199,7 +342,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
     double rs0;
-    if (!grid_sum<kCgThreads>(acc, w, rs0)) {
+    if (!grid_sum<THREADS>(acc, w, rs0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
@@ -219,7 +362,8 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
             acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
         double dq;
-        if (!grid_sum<kCgThreads>(acc, w, dq)) {
+        const bool ok = EXCH ? grid_exchange<THREADS>(acc, w, it + 1u, dq) : grid_sum<THREADS>(acc, w, dq);
+        if (!ok) {
             if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
             return;
         }
@@ -228,7 +372,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
             acc += phase_b_batch<kU>(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
         double rr;
-        if (!grid_sum<kCgThreads>(acc, w, rr)) {
+        if (!grid_sum<THREADS>(acc, w, rr)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
             return;
         }
@@ -244,7 +388,7 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
     acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<kU>(P, w.nmask, b, p, tw, ti, tile_id);
     double tr;
-    if (!grid_sum<kCgThreads>(acc, w, tr)) {
+    if (!grid_sum<THREADS>(acc, w, tr)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
@@ -258,22 +402,147 @@ __global__ __launch_bounds__(kCgThreads) void k_cg_persistent(StepParams P, cons
     }
 }
 
+// ---- single-reduction form ----
+// The same CG as k_cg_persistent in the Chronopoulos-Gear arrangement (one
+// sweep and one grid barrier per iteration, fs_cg.h).  In exact arithmetic
+// its iterates are those of the two-phase form; with f32 vectors they differ
+// by rounding.  Scalars per iteration k (f64, alpha/beta rounded to f32 as
+// src/sim/pressure.rs:282,296 round theirs):
+//     beta_k  = gamma_k / gamma_{k-1}                           (k > 0)
+//     alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//     alpha_0 = gamma_0 / delta_0
+// with gamma = r.r and delta = (A r).r; the stop test gamma_{k+1} <=
+// tol^2 gamma_0 is that of src/sim/pressure.rs:262-292.
+template <int THREADS, int U>
+__global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v, CgWork w) {
+    __shared__ uint32_t s_tiles[kLdsTiles];
+    const uint32_t G = gridDim.x;
+    const uint32_t nt = *w.n_tiles;
+    const uint32_t lb = xcd_remap(blockIdx.x, G);
+    const TileWalk tw = tile_walk(nt, lb, G, THREADS / 64);
+    const uint32_t s0 = (uint32_t)(((uint64_t)nt * lb) / G);
+    const uint32_t s1 = (uint32_t)(((uint64_t)nt * (lb + 1u)) / G);
+    const bool in_lds = s1 - s0 <= (uint32_t)kLdsTiles;
+    if (in_lds)
+        for (uint32_t i = s0 + threadIdx.x; i < s1; i += THREADS) s_tiles[i - s0] = w.tiles[i];
+    __syncthreads();
+    const uint32_t* gt = w.tiles;
+    auto tile_id = [&](uint32_t t) -> uint32_t { return in_lds ? s_tiles[t - s0] : gt[t]; };
+    const uint32_t step = tw.stride * U;
+
+    double part[2] = {0.0, 0.0}, tot[2];
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+        cgs_init_batch<U>(P, w.nmask, v.r[0], v.w[0], tw, ti, tile_id, part[0], part[1]);
+    if (!grid_sum_nv<THREADS, 2>(part, w, tot)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        return;
+    }
+    const double rs_init = tot[0];
+    const double tol = (double)P.tol;
+    const double tol_sq = tol * tol;
+    double gam = tot[0], del = tot[1], gam_prev = 0.0;
+    float alpha_prev = 0.0f;
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < P.iters; ++it) {
+        ++iters;
+        const bool first = it == 0;
+        float alpha, beta;
+        if (first) {
+            beta = 0.0f;
+            alpha = fabs(del) > 1e-30 ? (float)(gam / del) : 0.0f;
+        } else {
+            beta = gam_prev > 1e-30 ? (float)(gam / gam_prev) : 0.0f;
+            const double den = alpha_prev != 0.0f ? del - (double)beta * gam / (double)alpha_prev : del;
+            alpha = fabs(den) > 1e-30 ? (float)(gam / den) : 0.0f;
+        }
+        const uint32_t cur = it & 1u, nxt = cur ^ 1u;
+        const CgSweepIo io{v.r[cur], v.w[cur], v.s[nxt], v.r[nxt], v.w[nxt], v.s[cur], v.pd, v.x};
+        part[0] = 0.0;
+        part[1] = 0.0;
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+            cgs_sweep_batch<U>(P, w.nmask, io, alpha, beta, first, tw, ti, tile_id, part[0], part[1]);
+        if (!grid_sum_nv<THREADS, 2>(part, w, tot)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+            return;
+        }
+        gam_prev = gam;
+        alpha_prev = alpha;
+        gam = tot[0];
+        del = tot[1];
+        if (tol > 0.0 && gam <= tol_sq * rs_init) break;
+    }
+    double acc = 0.0;
+    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<U>(P, w.nmask, v.b, v.x, tw, ti, tile_id);
+    double tr;
+    if (!grid_exchange<THREADS>(acc, w, 1u, tr)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.state->rs_init = rs_init;
+        w.state->rs_old = gam;
+        w.state->true_res_sq = tr;
+        w.state->iterations = iters;
+        w.state->active = 0u;
+        w.state->error = 0u;
+    }
+}
+
+namespace {
+// Persistent-kernel variant: FLUIDSTEP_CG_FORM=twophase (default) or
+// single; workgroup size 1024 (default) or 512 (FLUIDSTEP_CG_THREADS);
+// for the two-phase form FLUIDSTEP_CG_EXCHANGE=0 keeps grid_sum after
+// phase A.
+struct PersistentVariant {
+    int threads;
+    bool single;
+    bool exch;
+    const void* fn;
+};
+PersistentVariant persistent_variant() {
+    static const PersistentVariant v = [] {
+        PersistentVariant r{1024, false, true, nullptr};
+        if (const char* e = std::getenv("FLUIDSTEP_CG_THREADS")) r.threads = std::atoi(e) == 512 ? 512 : 1024;
+        if (const char* e = std::getenv("FLUIDSTEP_CG_FORM")) r.single = std::strcmp(e, "single") == 0;
+        if (const char* e = std::getenv("FLUIDSTEP_CG_EXCHANGE")) r.exch = std::atoi(e) != 0;
+        if (r.single)
+            r.fn = r.threads == 1024 ? (const void*)k_cg_single<1024, 2> : (const void*)k_cg_single<512, 2>;
+        else if (r.threads == 1024)
+            r.fn = r.exch ? (const void*)k_cg_persistent<1024, true> : (const void*)k_cg_persistent<1024, false>;
+        else
+            r.fn = r.exch ? (const void*)k_cg_persistent<512, true> : (const void*)k_cg_persistent<512, false>;
+        return r;
+    }();
+    return v;
+}
+}  // namespace
+
+int cg_persistent_threads() { return persistent_variant().threads; }
+
 int cg_persistent_max_blocks() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const PersistentVariant v = persistent_variant();
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_persistent, kCgThreads, 0) != hipSuccess)
-        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v.fn, v.threads, 0) != hipSuccess) return 0;
     return cus * per_cu;
 }
 
-hipError_t launch_cg_persistent(const StepParams& P, const uint8_t* ct, const float* b, float* p, float* r,
-                                float* d0, float* d1, float* q, const CgWork& w, int grid_blocks, hipStream_t s) {
+hipError_t launch_cg_persistent(const StepParams& P, const CgVectors& v, const CgWork& w, int grid_blocks,
+                                hipStream_t s) {
+    const PersistentVariant pv = persistent_variant();
+    if (pv.single) {
+        void* args[] = {(void*)&P, (void*)&v, (void*)&w};
+        return hipLaunchCooperativeKernel(pv.fn, dim3(grid_blocks), dim3(pv.threads), args, 0, s);
+    }
+    const uint8_t* ct = nullptr;
+    const float* b = v.b;
+    float *p = v.x, *r = v.r[0], *d0 = v.pd, *d1 = v.s[0], *q = v.w[0];
     void* args[] = {(void*)&P, (void*)&ct, (void*)&b, (void*)&p, (void*)&r,
                     (void*)&d0, (void*)&d1, (void*)&q, (void*)&w};
-    return hipLaunchCooperativeKernel((const void*)k_cg_persistent, dim3(grid_blocks), dim3(kCgThreads), args, 0, s);
+    return hipLaunchCooperativeKernel(pv.fn, dim3(grid_blocks), dim3(pv.threads), args, 0, s);
 }
 
 // ---- multi-kernel form ----
