@@ -211,6 +211,7 @@ int enqueue_substep(fs_batch* bt, int cur, hipStream_t s) {
     FS_HIP(launch_compact_tiles(bt->tiles_tot + 1, bt->cg.tile_flags, bt->cg.tile_scan, bt->cg.tiles, bt->cg.n_tiles, s));
     FS_HIP(launch_fill_u32(bt->cg.arrive, 0u, 9 * 32, s));
     FS_HIP(launch_fill_u32(bt->cg.abort_flag, 0u, 1, s));
+    FS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(bt->cg.slots), 0u, bt->cg.slot_words, s));
     FS_HIP(launch_batch_cg(bt->d_tanks, bt->wl_cg, bt->n_cg, (int)bt->tuning.cg_unroll, bt->b, bt->p, bt->r, bt->d0,
                            bt->d1, bt->q, bt->cg, s));
     FS_HIP(launch_batch_gradient_enforce(bt->d_tanks, bt->wl_face, bt->n_face, bt->p, bt->ct, bt->grid, s));
@@ -404,13 +405,18 @@ FS_API int fs_batch_create(const fs_params* params, const float* const* pos_xyz,
         FS_HIP(bt->alloc(bt->cg.state, ntanks));
         bt->cg_max_blocks = batch_cg_max_blocks();
         if (bt->cg_max_blocks <= 0) return fail(FS_ERR_DEVICE, "cannot size the persistent CG grid");
-        bt->cg_default_blocks = half_resident_blocks(bt->cg_max_blocks);
+        int dev = 0, cus = 0;
+        FS_HIP(hipGetDevice(&dev));
+        FS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        bt->cg_default_blocks = persistent_grid_blocks(bt->cg_max_blocks, cus);
         if (const char* env = std::getenv("FLUIDSTEP_CG_BLOCKS")) {
             const int v = std::atoi(env);
             if (v > 0 && v <= bt->cg_max_blocks) bt->cg_default_blocks = v;
         }
         FS_HIP(bt->alloc(bt->cg.partials, 2 * ((size_t)bt->cg_max_blocks + 8)));
         FS_HIP(bt->alloc(bt->cg.flags, 2 * ((size_t)bt->cg_max_blocks + 8)));
+        bt->cg.slot_words = (uint32_t)(8 * ((size_t)bt->cg_max_blocks + 8));
+        FS_HIP(bt->alloc(bt->cg.slots, bt->cg.slot_words / 2));
         FS_HIP(bt->alloc(bt->d_tanks, ntanks));
         if (const char* env = std::getenv("FLUIDSTEP_BATCH_GRAPH")) bt->use_graph = std::atoi(env) != 0;
         // Work tables.
@@ -616,6 +622,7 @@ FS_API int fs_batch_autotune(fs_batch* bt, fs_batch_tuning* out) {
                                         bt->cg.n_tiles, s));
             FS_HIP(launch_fill_u32(bt->cg.arrive, 0u, 9 * 32, s));
             FS_HIP(launch_fill_u32(bt->cg.abort_flag, 0u, 1, s));
+            FS_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(bt->cg.slots), 0u, bt->cg.slot_words, s));
             double ms = 0.0;
             auto run = [&]() -> int {
                 FS_HIP(launch_batch_cg(bt->d_tanks, bt->wl_cg, bt->n_cg, u, bt->b, bt->p, bt->r, bt->d0, bt->d1,

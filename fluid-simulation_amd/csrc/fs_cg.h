@@ -44,6 +44,91 @@ __device__ __forceinline__ double sum_partials(const double* partials, uint32_t 
     return s_tree[0];
 }
 
+// Block partials of NV values: wave sums (lane 0's butterfly result), then
+// thread 0 adds the waves in order.  Valid in thread 0 only.
+template <int THREADS, int NV>
+__device__ __forceinline__ void block_sum_nv(const double (&v)[NV], double (&bs)[NV]) {
+    __shared__ double s_w[NV][THREADS / 64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const double t = wave_sum(v[j]);
+        if (lane == 0) s_w[j][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double t = 0.0;
+            for (int i = 0; i < THREADS / 64; ++i) t += s_w[j][i];
+            bs[j] = t;
+        }
+    }
+}
+
+// Total of G published partials in one fixed order: thread i has added
+// partials i, i + THREADS, ... into `loaded`; waves reduce by butterfly
+// (lane 0's result); thread 0 adds the wave results in order.  Every block
+// therefore forms the bit-identical total, returned to all threads.
+template <int THREADS, int NV>
+__device__ __forceinline__ void reduce_loaded(const double (&loaded)[NV], double (&out)[NV]) {
+    __shared__ double s_w[NV][THREADS / 64];
+    __shared__ double s_out[NV];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const double t = wave_sum(loaded[j]);
+        if (lane == 0) s_w[j][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double t = 0.0;
+            for (int i = 0; i < THREADS / 64; ++i) t += s_w[j][i];
+            s_out[j] = t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NV; ++j) out[j] = s_out[j];
+}
+
+// Epoch-tagged slots of the counter-free exchanges.  A slot is two 64-bit
+// words: the high half of an f64 under tag `epoch`, and the low half under
+// tag (epoch << 1 | flag).  Each word is written and read whole with an
+// agent-scope relaxed atomic (memory-side, around the per-XCD L2), so a
+// reader that sees both tags equal to the epoch it waits for holds that
+// epoch's value.  Epochs are below 2^31 and start at 1 over cleared slots.
+__device__ __forceinline__ void publish_tagged(uint64_t* half, uint32_t slot, uint32_t epoch, double v,
+                                               uint32_t flag) {
+    const uint64_t bits = (uint64_t)__double_as_longlong(v);
+    __hip_atomic_store(half + 2u * slot, ((uint64_t)epoch << 32) | (bits >> 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(half + 2u * slot + 1u, ((uint64_t)((epoch << 1) | (flag & 1u)) << 32) | (bits & 0xffffffffull),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits for slot `slot` to carry `epoch`; false (and the abort flag raised)
+// on abort or after kSpinTimeoutTicks.
+__device__ __forceinline__ bool poll_tagged(const uint64_t* half, uint32_t slot, uint32_t epoch, uint32_t* abort_flag,
+                                            double& v, uint32_t& flag) {
+    const long long t0 = wall_clock64();
+    for (uint32_t spin = 0;; ++spin) {
+        const uint64_t hi = __hip_atomic_load(half + 2u * slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t lo = __hip_atomic_load(half + 2u * slot + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)(hi >> 32) == epoch && (uint32_t)(lo >> 33) == epoch) {
+            v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+            flag = (uint32_t)(lo >> 32) & 1u;
+            return true;
+        }
+        if ((spin & 31u) == 31u && (ld_relaxed(abort_flag) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+            __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+    }
+}
+
 // Per-cell work, U tiles at a time: all loads of the batch are issued before
 // any result is computed, so a wave keeps several memory round trips in flight.
 struct TileWalk {

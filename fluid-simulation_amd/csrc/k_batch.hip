@@ -18,7 +18,7 @@
 namespace fs {
 
 namespace {
-constexpr int kBCgThreads = 512;   // CG workgroup (8 waves)
+constexpr int kBCgThreads = 1024;  // CG workgroup (16 waves), one per CU
 constexpr int kBLdsTiles = 2048;   // tile ids a CG workgroup keeps in LDS
 }  // namespace
 
@@ -248,10 +248,12 @@ hipError_t launch_batch_div_cginit(const BatchTank* T, const WorkItem* wl, uint3
 
 // ---------------------------------------------------------------------------
 // Lock-step reductions of the batched solve.  Every workgroup of the launch
-// meets every other at one grid barrier per reduction (two-level arrival as
+// meets every other at each reduction: the one after phase B (whose r the
+// next phase A reads at neighbours) is a grid barrier (two-level arrival as
 // in the single-tank solver: a counter per group of workgroups with equal
 // blockIdx % 8, then one over the groups; release RMWs, one acquire for the
-// last arrival, relaxed polling of the generation word by the others).  Each
+// last arrival, relaxed polling of the generation word by the others); the
+// one after phase A is the counter-free exchange (lockstep_exchange).  Each
 // workgroup publishes its f64 partial in its slot and whether its tank is
 // still iterating; afterwards it sums the slots of its own tank,
 // [base, base + nb), in slot order (so every workgroup of a tank holds the
@@ -277,19 +279,31 @@ __device__ __forceinline__ bool arrive_grid(const BatchCgWork& w) {
     return true;
 }
 
+// Per-tank total in one fixed order over the slots [base, base + nb): thread
+// i adds slot i, i + THREADS, ... when it lies in the range, then the block
+// reduces (reduce_loaded), so every workgroup of a tank holds the identical
+// total.
+template <int THREADS>
+__device__ __forceinline__ double tank_total(double loaded) {
+    const double in[1] = {loaded};
+    double o[1];
+    reduce_loaded<THREADS, 1>(in, o);
+    return o[0];
+}
+
 template <int THREADS>
 __device__ bool lockstep_sum(double v, uint32_t flag, const BatchCgWork& w, uint32_t slot, uint32_t base, uint32_t nb,
                              double& out, bool& any) {
-    __shared__ double s_w[THREADS / 64];
-    __shared__ double s_tree[THREADS];
     __shared__ int s_last, s_abort;
     __shared__ uint32_t s_gen;
-    const double bs = block_sum<THREADS>(v, s_w);
+    const double in[1] = {v};
+    double bs[1];
+    block_sum_nv<THREADS, 1>(in, bs);
     const uint32_t G = gridDim.x;
     if (threadIdx.x == 0) {
         const uint32_t g0 = ld_relaxed(w.gen);
         const size_t half = (size_t)(g0 & 1u) * G;
-        w.partials[half + slot] = bs;
+        w.partials[half + slot] = bs[0];
         w.flags[half + slot] = flag;
         s_gen = g0;
         s_abort = 0;
@@ -318,10 +332,53 @@ __device__ bool lockstep_sum(double v, uint32_t flag, const BatchCgWork& w, uint
     if (s_abort) return false;
     const size_t half = (size_t)(s_gen & 1u) * G;
     uint32_t f = 0;
-    for (uint32_t i = threadIdx.x; i < G; i += THREADS)
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < G; i += THREADS) {
         f |= __hip_atomic_load(&w.flags[half + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i >= base && i < base + nb)
+            acc += __hip_atomic_load(&w.partials[half + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     any = __syncthreads_or(f != 0u) != 0;
-    out = nb ? sum_partials<THREADS>(w.partials + half + base, nb, s_tree) : 0.0;
+    out = tank_total<THREADS>(acc);
+    return true;
+}
+
+// The reduction after phase A without counters or fences (phase B then reads
+// only data its own lanes wrote): every workgroup publishes its partial and
+// flag in its epoch-tagged slot (fs_cg.h) and reads all G slots; the tank
+// total and the OR of the flags are formed as in lockstep_sum.  Slots
+// alternate between two halves by epoch parity (a workgroup publishes e+2 only
+// after all have published e+1, i.e. finished reading e).
+template <int THREADS>
+__device__ bool lockstep_exchange(double v, uint32_t flag, const BatchCgWork& w, uint32_t slot, uint32_t base,
+                                  uint32_t nb, uint32_t epoch, double& out, bool& any) {
+    __shared__ int s_abort;
+    const uint32_t G = gridDim.x;
+    uint64_t* half = w.slots + (size_t)(epoch & 1u) * G * 2u;
+    const double in[1] = {v};
+    double bs[1];
+    block_sum_nv<THREADS, 1>(in, bs);
+    if (threadIdx.x == 0) {
+        s_abort = 0;
+        publish_tagged(half, slot, epoch, bs[0], flag);
+    }
+    __syncthreads();
+    uint32_t f = 0;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < G; i += THREADS) {
+        double x;
+        uint32_t fi;
+        if (!poll_tagged(half, i, epoch, w.abort_flag, x, fi)) {
+            s_abort = 1;
+            break;
+        }
+        f |= fi;
+        if (i >= base && i < base + nb) acc += x;
+    }
+    any = __syncthreads_or(f != 0u) != 0;
+    const double tot = tank_total<THREADS>(acc);
+    if (s_abort) return false;
+    out = tot;
     return true;
 }
 
@@ -403,7 +460,7 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
                 acc += phase_a_batch<U>(P, nm, rr_, dprev, dnext, q_, beta, first, tw, ti, tile_id);
         }
         double dq;
-        if (!lockstep_sum<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, dq, any)) {
+        if (!lockstep_exchange<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, it + 1u, dq, any)) {
             report(rs_init, rs_old, 0.0, iters, 1u);
             return;
         }

@@ -127,56 +127,6 @@ __device__ __forceinline__ bool arrive(const CgWork& w) {
     return true;
 }
 
-// Block partials of NV values: wave sums (lane 0's butterfly result), then
-// thread 0 adds the waves in order.  Valid in thread 0 only.
-template <int THREADS, int NV>
-__device__ __forceinline__ void block_sum_nv(const double (&v)[NV], double (&bs)[NV]) {
-    __shared__ double s_w[NV][THREADS / 64];
-    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-        const double t = wave_sum(v[j]);
-        if (lane == 0) s_w[j][wv] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            double t = 0.0;
-            for (int i = 0; i < THREADS / 64; ++i) t += s_w[j][i];
-            bs[j] = t;
-        }
-    }
-}
-
-// Total of G published partials in one fixed order: thread i has added
-// partials i, i + THREADS, ... into `loaded`; waves reduce by butterfly
-// (lane 0's result); thread 0 adds the wave results in order.  Every block
-// therefore forms the bit-identical total, returned to all threads.
-template <int THREADS, int NV>
-__device__ __forceinline__ void reduce_loaded(const double (&loaded)[NV], double (&out)[NV]) {
-    __shared__ double s_w[NV][THREADS / 64];
-    __shared__ double s_out[NV];
-    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-        const double t = wave_sum(loaded[j]);
-        if (lane == 0) s_w[j][wv] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            double t = 0.0;
-            for (int i = 0; i < THREADS / 64; ++i) t += s_w[j][i];
-            s_out[j] = t;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < NV; ++j) out[j] = s_out[j];
-}
-
 // Grid barrier + sum for the persistent kernels.  Orders memory: every
 // block's stores before the barrier are visible to every block after it.
 // The last arrival only bumps the generation word; afterwards every block
@@ -244,49 +194,36 @@ __device__ __forceinline__ bool grid_sum(double v, const CgWork& w, double& out)
 }
 
 // Reduction without counters or fences, for a total whose consumers read no
-// data another block wrote since the previous grid_sum.  Every block stores
-// its partial as two 64-bit words, each tagged with the epoch in its upper
-// half (high and low halves of the f64 in the lower), in its own slot, and
-// reads all G slots until each carries the epoch; the total is then formed
-// in grid_sum's order.  Tagged words are written and read with agent-scope
-// relaxed atomics, which go to memory around the per-XCD L2, so no
-// write-back or invalidate is issued.  Slots alternate between two halves by
-// epoch parity: a block publishes epoch e+1 only after reading every epoch-e
-// slot, and epoch e+2 only after every block has published e+1, i.e.
-// finished reading epoch e.  The slots start cleared (tag 0) and epochs
-// start at 1.  Returns false on abort.
+// data another block wrote since the previous grid_sum: every block publishes
+// its partial in its own epoch-tagged slot (fs_cg.h) and reads all G slots
+// until each carries the epoch; the total is then formed in grid_sum's order.
+// No write-back or invalidate is issued.  Slots alternate between two halves
+// by epoch parity: a block publishes epoch e+1 only after reading every
+// epoch-e slot, and epoch e+2 only after every block has published e+1, i.e.
+// finished reading epoch e.  The slots start cleared and epochs start at 1.
+// Returns false on abort.
 template <int THREADS>
 __device__ bool grid_exchange(double v, const CgWork& w, uint32_t epoch, double& out) {
     __shared__ int s_abort;
     const uint32_t G = gridDim.x;
     uint64_t* half = w.slots + (size_t)(epoch & 1u) * G * 2u;
-    const uint64_t tag = (uint64_t)epoch << 32;
     const double in[1] = {v};
     double bs[1];
     block_sum_nv<THREADS, 1>(in, bs);
     if (threadIdx.x == 0) {
         s_abort = 0;
-        const uint64_t bits = (uint64_t)__double_as_longlong(bs[0]);
-        __hip_atomic_store(half + 2u * blockIdx.x, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(half + 2u * blockIdx.x + 1u, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        publish_tagged(half, blockIdx.x, epoch, bs[0], 0u);
     }
     __syncthreads();
     double acc[1] = {0.0};
     for (uint32_t i = threadIdx.x; i < G; i += THREADS) {
-        const long long t0 = wall_clock64();
-        uint64_t hi, lo;
-        for (uint32_t spin = 0;; ++spin) {
-            hi = __hip_atomic_load(half + 2u * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            lo = __hip_atomic_load(half + 2u * i + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((uint32_t)(hi >> 32) == epoch && (uint32_t)(lo >> 32) == epoch) break;
-            if ((spin & 31u) == 31u && (ld_relaxed(w.abort_flag) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
-                __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_abort = 1;
-                break;
-            }
+        double x;
+        uint32_t f;
+        if (!poll_tagged(half, i, epoch, w.abort_flag, x, f)) {
+            s_abort = 1;
+            break;
         }
-        acc[0] += __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+        acc[0] += x;
     }
     double o[1];
     reduce_loaded<THREADS, 1>(acc, o);
