@@ -231,6 +231,105 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
     return acc;
 }
 
+// ---- register-resident form of the two phases ----
+// When a wave's tiles fit in one batch of U, its cells' mask, r, d, p and q
+// stay in registers for the whole solve: phase A loads only the neighbours'
+// r and d and stores d (read by neighbours next iteration); phase B loads
+// nothing and stores r; p is stored once after the loop.  The f32
+// expressions are those of phase_a_batch / phase_b_batch, so both forms give
+// the same bits.
+template <int U>
+struct ResidentCells {
+    uint32_t c[U];
+    uint16_t m[U];
+    float r[U], d[U], p[U], q[U];
+};
+
+template <int U, typename TileId>
+__device__ __forceinline__ double resident_load(const StepParams& P, const uint16_t* __restrict__ nmask,
+                                                const float* __restrict__ r, const float* __restrict__ d,
+                                                const float* __restrict__ p, const TileWalk& tw, TileId&& tile_id,
+                                                ResidentCells<U>& rc) {
+    const uint32_t lane = lane_id(), C = P.C;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = tw.begin + (uint32_t)u * tw.stride;
+        rc.c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
+        const bool ok = rc.c[u] < C;
+        rc.m[u] = ok ? nmask[rc.c[u]] : (uint16_t)0;
+        rc.r[u] = ok ? r[rc.c[u]] : 0.0f;
+        rc.d[u] = ok ? d[rc.c[u]] : 0.0f;
+        rc.p[u] = ok ? p[rc.c[u]] : 0.0f;
+        rc.q[u] = 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (rc.c[u] < C) acc += (double)rc.r[u] * (double)rc.r[u];
+    return acc;
+}
+
+template <int U>
+__device__ __forceinline__ double resident_phase_a(const StepParams& P, const float* __restrict__ r,
+                                                   const float* __restrict__ dprev, float* __restrict__ dnext,
+                                                   float beta, bool first, ResidentCells<U>& rc) {
+    const uint32_t C = P.C;
+    const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
+    float rn[U][6], dnb[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        // The cell index is made opaque here (an empty asm, no instruction) so
+        // the 6 x U neighbour addresses are formed per iteration instead of
+        // being hoisted out of the solve loop and held in registers throughout.
+        uint32_t cu = rc.c[u];
+        asm volatile("" : "+v"(cu));
+        const bool ok = cu < C;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            int32_t nb = (int32_t)cu + off[s];
+            nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
+            rn[u][s] = (ok && !first) ? r[nb] : 0.0f;
+            dnb[u][s] = ok ? dprev[nb] : 0.0f;
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (rc.c[u] >= C) continue;
+        const uint16_t m = rc.m[u];
+        const float dn = first ? rc.d[u] : rc.r[u] + beta * rc.d[u];
+        float qc = 0.0f;
+        if (m & kMaskLiquid) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                if ((m >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
+            qc = (float)(m & 7u) * dn - sum;
+        }
+        if (!first) dnext[rc.c[u]] = dn;
+        rc.d[u] = dn;
+        rc.q[u] = qc;
+        acc += (double)dn * (double)qc;
+    }
+    return acc;
+}
+
+template <int U>
+__device__ __forceinline__ double resident_phase_b(const StepParams& P, float* __restrict__ r, float alpha,
+                                                   ResidentCells<U>& rc) {
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (rc.c[u] >= P.C) continue;
+        rc.p[u] = rc.p[u] + alpha * rc.d[u];
+        const float rn = rc.r[u] - alpha * rc.q[u];
+        r[rc.c[u]] = rn;
+        rc.r[u] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    return acc;
+}
+
 // ---- single-reduction form (Chronopoulos & Gear 1989) ----
 // With s = A pd and w = A r kept as vectors, one sweep per iteration does
 //     pd = r + beta pd;  s = w + beta s;  x += alpha pd;  r' = r - alpha s;

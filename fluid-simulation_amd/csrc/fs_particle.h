@@ -20,31 +20,37 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
     const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
     const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
     const int32_t lx = a == 0 ? 1 : 0, ly = a == 1 ? 1 : 0, lz = a == 2 ? 1 : 0;
+    // All 16 loads are issued first, at index 0 for a skipped tap (so every
+    // address is in bounds), then the taps are accumulated in the loop order
+    // of the reference; a skipped tap leaves the sums unchanged (a select,
+    // not an added zero), so the result equals the branchy form bit for bit.
+    bool ok[8];
+    uint32_t idx[8];
+    float fv[8], sw[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int32_t ii = bx + (t & 1), jj = by + ((t >> 1) & 1), kk = bz + (t >> 2);
+        // in range, and not *_face_touches_static_solid (sh/g2p.wgsl:35-54)
+        ok[t] = ii >= 0 && ii < (int32_t)dim[0] && jj >= 0 && jj < (int32_t)dim[1] && kk >= 0 && kk < (int32_t)dim[2] &&
+                !(ii <= lx || ii >= nx - 1 || jj <= ly || jj >= ny - 1 || kk <= lz || kk >= nz - 1);
+        idx[t] = ok[t] ? (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk) : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        fv[t] = vel[idx[t]];
+        sw[t] = sav[idx[t]];
+    }
     float fin = 0.0f, sv = 0.0f, wsum = 0.0f;
 #pragma unroll
-    for (int dk = 0; dk < 2; ++dk) {
-        const int32_t kk = bz + dk;
-        if (kk < 0 || kk >= (int32_t)dim[2]) continue;
+    for (int t = 0; t < 8; ++t) {
+        const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
         const float wz = dk == 1 ? tz : 1.0f - tz;
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj) {
-            const int32_t jj = by + dj;
-            if (jj < 0 || jj >= (int32_t)dim[1]) continue;
-            const float wy = dj == 1 ? ty : 1.0f - ty;
-#pragma unroll
-            for (int di = 0; di < 2; ++di) {
-                const int32_t ii = bx + di;
-                if (ii < 0 || ii >= (int32_t)dim[0]) continue;
-                // *_face_touches_static_solid (sh/g2p.wgsl:35-54)
-                if (ii <= lx || ii >= nx - 1 || jj <= ly || jj >= ny - 1 || kk <= lz || kk >= nz - 1) continue;
-                const float wx = di == 1 ? tx : 1.0f - tx;
-                const float w = wx * wy * wz;
-                const uint32_t idx = (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk);
-                fin += w * vel[idx];
-                sv += w * sav[idx];
-                wsum += w;
-            }
-        }
+        const float wy = dj == 1 ? ty : 1.0f - ty;
+        const float wx = di == 1 ? tx : 1.0f - tx;
+        const float w = wx * wy * wz;
+        fin = ok[t] ? fin + w * fv[t] : fin;
+        sv = ok[t] ? sv + w * sw[t] : sv;
+        wsum = ok[t] ? wsum + w : wsum;
     }
     if (wsum > 0.0f) {
         ofin = fin / wsum;

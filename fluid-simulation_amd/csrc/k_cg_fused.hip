@@ -251,6 +251,73 @@ __device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
 }
 
 // ---- persistent form ----
+// The iteration loop of the two-phase solve over one block's tiles; RES
+// keeps the cells in registers (every wave has at most one batch of kU
+// tiles).  Returns false on abort.  Both forms run the same reductions in
+// the same order and the same f32 expressions.
+template <int THREADS, bool EXCH, bool RES, typename TileId>
+__device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* __restrict__ b, float* __restrict__ p,
+                                             float* __restrict__ r, float* __restrict__ d0, float* __restrict__ d1,
+                                             float* __restrict__ q, const CgWork& w, const TileWalk& tw,
+                                             TileId&& tile_id, double& rs_init_out, double& rs_final,
+                                             uint32_t& iters) {
+    const uint32_t step = tw.stride * kU;
+    ResidentCells<kU> rc;
+    double acc = 0.0;
+    if (RES)
+        acc = resident_load<kU>(P, w.nmask, r, d0, p, tw, tile_id, rc);
+    else
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
+    double rs0;
+    if (!grid_sum<THREADS>(acc, w, rs0)) return false;
+    double rs_old = rs0;
+    const double rs_init = rs0;
+    const double tol = (double)P.tol;
+    const double tol_sq = tol * tol;
+    float beta = 0.0f;
+    float* dprev = d0;
+    float* dnext = d1;
+    iters = 0;
+    for (uint32_t it = 0; it < P.iters; ++it) {
+        ++iters;
+        const bool first = it == 0;
+        float* dn_buf = first ? dprev : dnext;
+        acc = 0.0;
+        if (RES)
+            acc = resident_phase_a<kU>(P, r, dprev, dnext, beta, first, rc);
+        else
+            for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
+        double dq;
+        const bool ok = EXCH ? grid_exchange<THREADS>(acc, w, it + 1u, dq) : grid_sum<THREADS>(acc, w, dq);
+        if (!ok) return false;
+        const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
+        acc = 0.0;
+        if (RES)
+            acc = resident_phase_b<kU>(P, r, alpha, rc);
+        else
+            for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                acc += phase_b_batch<kU>(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
+        double rr;
+        if (!grid_sum<THREADS>(acc, w, rr)) return false;
+        beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
+        rs_old = rr;
+        if (tol > 0.0 && rr <= tol_sq * rs_init) break;
+        if (!first) {
+            float* t = dprev;
+            dprev = dnext;
+            dnext = t;
+        }
+    }
+    if (RES)
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (rc.c[u] < P.C) p[rc.c[u]] = rc.p[u];
+    rs_init_out = rs_init;
+    rs_final = rs_old;
+    return true;
+}
+
 // THREADS: workgroup size; EXCH: the reduction after phase A (dn.q, which
 // phase B consumes with data only its own lanes wrote) uses grid_exchange.
 template <int THREADS, bool EXCH>
@@ -275,57 +342,31 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     const uint32_t* gt = w.tiles;
     auto tile_id = [&](uint32_t t) -> uint32_t { return in_lds ? s_tiles[t - s0] : gt[t]; };
     const uint32_t step = tw.stride * kU;
-
-    double acc = 0.0;
-    for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
-    double rs0;
-    if (!grid_sum<THREADS>(acc, w, rs0)) {
+    // A block whose slice gives every wave at most one batch of kU tiles keeps
+    // its cells in registers; the others stream the vectors.
+    const bool resident = s1 - s0 <= (uint32_t)(THREADS / 64 * kU);
+    double rs_init = 0.0, rs_old = 0.0;
+    uint32_t iters = 0;
+    const bool ok = resident
+                        ? cg_two_phase<THREADS, EXCH, true>(P, b, p, r, d0, d1, q, w, tw, tile_id, rs_init, rs_old, iters)
+                        : cg_two_phase<THREADS, EXCH, false>(P, b, p, r, d0, d1, q, w, tw, tile_id, rs_init, rs_old,
+                                                             iters);
+    if (!ok) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
-    double rs_old = rs0;
-    const double rs_init = rs0;
-    const double tol = (double)P.tol;
-    const double tol_sq = tol * tol;
-    float beta = 0.0f;
-    float* dprev = d0;
-    float* dnext = d1;
-    uint32_t iters = 0;
-    for (uint32_t it = 0; it < P.iters; ++it) {
-        ++iters;
-        const bool first = it == 0;
-        float* dn_buf = first ? dprev : dnext;
-        acc = 0.0;
-        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-            acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
-        double dq;
-        const bool ok = EXCH ? grid_exchange<THREADS>(acc, w, it + 1u, dq) : grid_sum<THREADS>(acc, w, dq);
-        if (!ok) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
-            return;
-        }
-        const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
-        acc = 0.0;
-        for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-            acc += phase_b_batch<kU>(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
-        double rr;
-        if (!grid_sum<THREADS>(acc, w, rr)) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
-            return;
-        }
-        beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
-        rs_old = rr;
-        if (tol > 0.0 && rr <= tol_sq * rs_init) break;
-        if (!first) {
-            float* t = dprev;
-            dprev = dnext;
-            dnext = t;
-        }
+    // Resident blocks stored p after the loop: one more barrier publishes it
+    // before the true residual reads neighbours' p.
+    double unused;
+    if (!grid_sum<THREADS>(0.0, w, unused)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        return;
     }
-    acc = 0.0;
+    double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<kU>(P, w.nmask, b, p, tw, ti, tile_id);
     double tr;
-    if (!grid_sum<THREADS>(acc, w, tr)) {
+    const bool ok2 = EXCH ? grid_exchange<THREADS>(acc, w, iters + 1u, tr) : grid_sum<THREADS>(acc, w, tr);
+    if (!ok2) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }

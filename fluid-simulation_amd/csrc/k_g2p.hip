@@ -18,7 +18,9 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, ParticleSoA sp, const
                                               const float* __restrict__ v, const float* __restrict__ w,
                                               const float* __restrict__ us, const float* __restrict__ vs,
                                               const float* __restrict__ ws) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    // Each XCD takes a contiguous eighth of the sorted particles, so the face
+    // lines a region of particles reads are fetched into one XCD's L2.
+    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= P.N) return;
     g2p_particle<BF16>(P, sp, p, u, v, w, us, vs, ws);
 }
@@ -26,12 +28,13 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, ParticleSoA sp, const
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16, const float* const vel[3],
                       const float* const saved[3], hipStream_t s) {
     if (P.N == 0) return hipSuccess;
+    const unsigned G = (grid_for(P.N, 256) + 7u) / 8u * 8u;  // whole XCD groups for xcd_remap
     if (bf16)
-        hipLaunchKernelGGL(k_g2p<1>, dim3(grid_for(P.N, 256)), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2],
-                           saved[0], saved[1], saved[2]);
+        hipLaunchKernelGGL(k_g2p<1>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
+                           saved[2]);
     else
-        hipLaunchKernelGGL(k_g2p<0>, dim3(grid_for(P.N, 256)), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2],
-                           saved[0], saved[1], saved[2]);
+        hipLaunchKernelGGL(k_g2p<0>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
+                           saved[2]);
     return hipGetLastError();
 }
 
