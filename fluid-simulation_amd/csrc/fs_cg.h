@@ -238,6 +238,14 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
 // nothing and stores r; p is stored once after the loop.  The f32
 // expressions are those of phase_a_batch / phase_b_batch, so both forms give
 // the same bits.
+// Store of a value other blocks read after the next grid barrier: an
+// agent-scope relaxed atomic store goes through the XCD's L2 to memory at
+// once, so the barrier's release finds the line clean instead of writing
+// it back then.
+__device__ __forceinline__ void store_through(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int U>
 struct ResidentCells {
     uint32_t c[U];
@@ -306,7 +314,7 @@ __device__ __forceinline__ double resident_phase_a(const StepParams& P, const fl
                 if ((m >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
             qc = (float)(m & 7u) * dn - sum;
         }
-        if (!first) dnext[rc.c[u]] = dn;
+        if (!first) store_through(&dnext[rc.c[u]], dn);
         rc.d[u] = dn;
         rc.q[u] = qc;
         acc += (double)dn * (double)qc;
@@ -323,7 +331,7 @@ __device__ __forceinline__ double resident_phase_b(const StepParams& P, float* _
         if (rc.c[u] >= P.C) continue;
         rc.p[u] = rc.p[u] + alpha * rc.d[u];
         const float rn = rc.r[u] - alpha * rc.q[u];
-        r[rc.c[u]] = rn;
+        store_through(&r[rc.c[u]], rn);
         rc.r[u] = rn;
         acc += (double)rn * (double)rn;
     }

@@ -110,8 +110,7 @@ hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uin
 // Arrivals are release RMWs; only a block that turns out to be a group's or
 // the grid's last arrival issues the acquire fence (fence-after-RMW
 // synchronizes with every release earlier in the counter's RMW chain).
-__device__ __forceinline__ bool arrive(const CgWork& w) {
-    const uint32_t G = gridDim.x;
+__device__ __forceinline__ bool arrive(const CgWork& w, uint32_t G) {
     const uint32_t grp = blockIdx.x & 7u;
     const uint32_t in_grp = (G - grp + 7u) / 8u;
     const uint32_t ngrp = G < 8u ? G : 8u;
@@ -136,19 +135,18 @@ __device__ __forceinline__ bool arrive(const CgWork& w) {
 // publication cannot overwrite a value a slow block is still summing.
 // Returns false on abort.
 template <int THREADS, int NV>
-__device__ bool grid_sum_nv(const double (&v)[NV], const CgWork& w, double (&out)[NV]) {
+__device__ bool grid_sum_nv(const double (&v)[NV], const CgWork& w, uint32_t G, double (&out)[NV]) {
     __shared__ int s_last, s_abort;
     __shared__ uint32_t s_gen;
     double bs[NV];
     block_sum_nv<THREADS, NV>(v, bs);
-    const uint32_t G = gridDim.x;
     if (threadIdx.x == 0) {
         const uint32_t g0 = ld_relaxed(w.bar_gen);
 #pragma unroll
         for (int j = 0; j < NV; ++j) w.partials[((size_t)(g0 & 1u) * G + blockIdx.x) * NV + j] = bs[j];
         s_gen = g0;
         s_abort = 0;
-        s_last = arrive(w) ? 1 : 0;
+        s_last = arrive(w, G) ? 1 : 0;
     }
     __syncthreads();
     if (s_last) {
@@ -185,10 +183,10 @@ __device__ bool grid_sum_nv(const double (&v)[NV], const CgWork& w, double (&out
 }
 
 template <int THREADS>
-__device__ __forceinline__ bool grid_sum(double v, const CgWork& w, double& out) {
+__device__ __forceinline__ bool grid_sum(double v, const CgWork& w, uint32_t G, double& out) {
     const double in[1] = {v};
     double o[1];
-    if (!grid_sum_nv<THREADS, 1>(in, w, o)) return false;
+    if (!grid_sum_nv<THREADS, 1>(in, w, G, o)) return false;
     out = o[0];
     return true;
 }
@@ -203,9 +201,8 @@ __device__ __forceinline__ bool grid_sum(double v, const CgWork& w, double& out)
 // finished reading epoch e.  The slots start cleared and epochs start at 1.
 // Returns false on abort.
 template <int THREADS>
-__device__ bool grid_exchange(double v, const CgWork& w, uint32_t epoch, double& out) {
+__device__ bool grid_exchange(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
     __shared__ int s_abort;
-    const uint32_t G = gridDim.x;
     uint64_t* half = w.slots + (size_t)(epoch & 1u) * G * 2u;
     const double in[1] = {v};
     double bs[1];
@@ -242,7 +239,7 @@ __device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
     const double bs = block_sum<THREADS>(v, s_w);
     if (threadIdx.x == 0) {
         w.partials[blockIdx.x] = bs;
-        s_last = arrive(w) ? 1 : 0;
+        s_last = arrive(w, gridDim.x) ? 1 : 0;
     }
     __syncthreads();
     if (!s_last) return false;
@@ -251,6 +248,20 @@ __device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
 }
 
 // ---- persistent form ----
+// Workgroups that take part in a persistent solve over nt liquid tiles: the
+// fewest whole XCD groups (multiples of 8) that leave each workgroup at most
+// per_block tiles, capped at the launched grid.  A grid barrier's latency
+// grows with the number of arrivals, so a small solve runs on few
+// workgroups; the others exit at once and never meet a barrier.  Every
+// workgroup reads the same nt, so all agree on the count.
+__device__ __forceinline__ uint32_t solve_blocks(uint32_t nt, uint32_t per_block) {
+    const uint32_t launched = gridDim.x;
+    uint32_t g = (nt + per_block - 1u) / per_block;
+    g = (g + 7u) / 8u * 8u;
+    if (g < 8u) g = 8u;
+    return g < launched ? g : launched;
+}
+
 // The iteration loop of the two-phase solve over one block's tiles; RES
 // keeps the cells in registers (every wave has at most one batch of kU
 // tiles).  Returns false on abort.  Both forms run the same reductions in
@@ -258,7 +269,7 @@ __device__ bool last_block_sum(double v, const CgWork& w, double& tot) {
 template <int THREADS, bool EXCH, bool RES, typename TileId>
 __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* __restrict__ b, float* __restrict__ p,
                                              float* __restrict__ r, float* __restrict__ d0, float* __restrict__ d1,
-                                             float* __restrict__ q, const CgWork& w, const TileWalk& tw,
+                                             float* __restrict__ q, const CgWork& w, uint32_t G, const TileWalk& tw,
                                              TileId&& tile_id, double& rs_init_out, double& rs_final,
                                              uint32_t& iters) {
     const uint32_t step = tw.stride * kU;
@@ -269,7 +280,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
     else
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
     double rs0;
-    if (!grid_sum<THREADS>(acc, w, rs0)) return false;
+    if (!grid_sum<THREADS>(acc, w, G, rs0)) return false;
     double rs_old = rs0;
     const double rs_init = rs0;
     const double tol = (double)P.tol;
@@ -289,7 +300,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
                 acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
         double dq;
-        const bool ok = EXCH ? grid_exchange<THREADS>(acc, w, it + 1u, dq) : grid_sum<THREADS>(acc, w, dq);
+        const bool ok = EXCH ? grid_exchange<THREADS>(acc, w, G, it + 1u, dq) : grid_sum<THREADS>(acc, w, G, dq);
         if (!ok) return false;
         const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
         acc = 0.0;
@@ -299,7 +310,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
                 acc += phase_b_batch<kU>(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
         double rr;
-        if (!grid_sum<THREADS>(acc, w, rr)) return false;
+        if (!grid_sum<THREADS>(acc, w, G, rr)) return false;
         beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
         rs_old = rr;
         if (tol > 0.0 && rr <= tol_sq * rs_init) break;
@@ -328,8 +339,9 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
                                                                CgWork w) {
     __shared__ uint32_t s_tiles[kLdsTiles];
     (void)ct;
-    const uint32_t G = gridDim.x;
     const uint32_t nt = *w.n_tiles;
+    const uint32_t G = solve_blocks(nt, THREADS / 64 * kU);
+    if (blockIdx.x >= G) return;
     const uint32_t lb = xcd_remap(blockIdx.x, G);
     const TileWalk tw = tile_walk(nt, lb, G, THREADS / 64);
     // This block's slice of the tile list, kept in LDS when it fits.
@@ -348,9 +360,10 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     double rs_init = 0.0, rs_old = 0.0;
     uint32_t iters = 0;
     const bool ok = resident
-                        ? cg_two_phase<THREADS, EXCH, true>(P, b, p, r, d0, d1, q, w, tw, tile_id, rs_init, rs_old, iters)
-                        : cg_two_phase<THREADS, EXCH, false>(P, b, p, r, d0, d1, q, w, tw, tile_id, rs_init, rs_old,
-                                                             iters);
+                        ? cg_two_phase<THREADS, EXCH, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
+                                                            iters)
+                        : cg_two_phase<THREADS, EXCH, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
+                                                             rs_old, iters);
     if (!ok) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
@@ -358,14 +371,14 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     // Resident blocks stored p after the loop: one more barrier publishes it
     // before the true residual reads neighbours' p.
     double unused;
-    if (!grid_sum<THREADS>(0.0, w, unused)) {
+    if (!grid_sum<THREADS>(0.0, w, G, unused)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<kU>(P, w.nmask, b, p, tw, ti, tile_id);
     double tr;
-    const bool ok2 = EXCH ? grid_exchange<THREADS>(acc, w, iters + 1u, tr) : grid_sum<THREADS>(acc, w, tr);
+    const bool ok2 = EXCH ? grid_exchange<THREADS>(acc, w, G, iters + 1u, tr) : grid_sum<THREADS>(acc, w, G, tr);
     if (!ok2) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
@@ -394,8 +407,9 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
 template <int THREADS, int U>
 __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v, CgWork w) {
     __shared__ uint32_t s_tiles[kLdsTiles];
-    const uint32_t G = gridDim.x;
     const uint32_t nt = *w.n_tiles;
+    const uint32_t G = solve_blocks(nt, THREADS / 64 * kU);
+    if (blockIdx.x >= G) return;
     const uint32_t lb = xcd_remap(blockIdx.x, G);
     const TileWalk tw = tile_walk(nt, lb, G, THREADS / 64);
     const uint32_t s0 = (uint32_t)(((uint64_t)nt * lb) / G);
@@ -411,7 +425,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v
     double part[2] = {0.0, 0.0}, tot[2];
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
         cgs_init_batch<U>(P, w.nmask, v.r[0], v.w[0], tw, ti, tile_id, part[0], part[1]);
-    if (!grid_sum_nv<THREADS, 2>(part, w, tot)) {
+    if (!grid_sum_nv<THREADS, 2>(part, w, G, tot)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
@@ -439,7 +453,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v
         part[1] = 0.0;
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
             cgs_sweep_batch<U>(P, w.nmask, io, alpha, beta, first, tw, ti, tile_id, part[0], part[1]);
-        if (!grid_sum_nv<THREADS, 2>(part, w, tot)) {
+        if (!grid_sum_nv<THREADS, 2>(part, w, G, tot)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
             return;
         }
@@ -452,7 +466,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<U>(P, w.nmask, v.b, v.x, tw, ti, tile_id);
     double tr;
-    if (!grid_exchange<THREADS>(acc, w, 1u, tr)) {
+    if (!grid_exchange<THREADS>(acc, w, G, 1u, tr)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
