@@ -112,13 +112,35 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
             const int32_t lx_dim = a == 0 ? kGBX + 1 : kGBX;
             const int32_t ly_dim = a == 1 ? kGBY + 1 : kGBY;
             const int32_t base = a == 0 ? 0 : (a == 1 ? kGU : kGU + kGV);
-            p2g_taps(g, dim, pv[a], P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
-                const int32_t lx = ii - x0, ly = jj - y0, lz = kk - z0;
-                if (lx < 0 || lx > hx || ly < 0 || ly > hy || lz < 0 || lz > hz) return;
-                const int32_t li = base + lx + lx_dim * (ly + ly_dim * lz);
-                atomicAdd(&s_num[li], tn);
-                atomicAdd(&s_den[li], td);
-            });
+            // The taps of p2g_taps (sh/scatter.wgsl:78-118) as straight-line
+            // code: per dimension, whether each of the two face indices is in
+            // the grid and owned by this brick; a tap is added when all three
+            // hold and its weight is positive.  Same expressions, same sums.
+            const int32_t bx = sat_i32(floorf(g[0])), by = sat_i32(floorf(g[1])), bz = sat_i32(floorf(g[2]));
+            const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
+            bool ox[2], oy[2], oz[2];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const int32_t ii = bx + d, jj = by + d, kk = bz + d;
+                ox[d] = ii >= 0 && ii < (int32_t)dim[0] && ii - x0 >= 0 && ii - x0 <= hx;
+                oy[d] = jj >= 0 && jj < (int32_t)dim[1] && jj - y0 >= 0 && jj - y0 <= hy;
+                oz[d] = kk >= 0 && kk < (int32_t)dim[2] && kk - z0 >= 0 && kk - z0 <= hz;
+            }
+            if (!((ox[0] || ox[1]) && (oy[0] || oy[1]) && (oz[0] || oz[1]))) continue;
+            const float vcomp = pv[a];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
+                const float wz = dk == 1 ? tz : 1.0f - tz;
+                const float wy = dj == 1 ? ty : 1.0f - ty;
+                const float wx = di == 1 ? tx : 1.0f - tx;
+                const float w = wx * wy * wz;
+                if (ox[di] && oy[dj] && oz[dk] && w > 0.0f) {
+                    const int32_t li = base + (bx + di - x0) + lx_dim * ((by + dj - y0) + ly_dim * (bz + dk - z0));
+                    atomicAdd(&s_num[li], sat_i32(rintf(w * vcomp * P.scale)));
+                    atomicAdd(&s_den[li], sat_i32(rintf(w * P.scale)));
+                }
+            }
         }
     };
 
