@@ -6,6 +6,9 @@
 
 namespace fs {
 
+// Two f32 at 4-byte alignment (gfx950 loads them with one 8-byte access).
+typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+
 // sample_u/v/w of sh/g2p.wgsl:56-129: wall-aware trilinear gather of
 // (final, saved) with renormalized weights.
 __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const float Pp[3],
@@ -20,34 +23,52 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
     const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
     const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
     const int32_t lx = a == 0 ? 1 : 0, ly = a == 1 ? 1 : 0, lz = a == 2 ? 1 : 0;
-    // All 16 loads are issued first, at index 0 for a skipped tap (so every
-    // address is in bounds), then the taps are accumulated in the loop order
-    // of the reference; a skipped tap leaves the sums unchanged (a select,
-    // not an added zero), so the result equals the branchy form bit for bit.
+    // Tap (di, dj, dk) is used when its face is in the grid and does not
+    // touch a static solid (*_face_touches_static_solid, sh/g2p.wgsl:35-54).
+    // Both tests are per dimension: 0 <= i < dim and !(i <= l || i >= n-1)
+    // reduce to l < i < n-1.  All loads are issued first, at offset 0 for a
+    // skipped row (in bounds), then the taps are accumulated in the loop order
+    // of the reference; a skipped tap leaves the sums unchanged (a select, not
+    // an added zero), so the result equals the branchy form bit for bit.
+    // Offsets are 32-bit bytes from the array base.
+    bool okx[2], oky[2], okz[2];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        okx[d] = bx + d > lx && bx + d < nx - 1;
+        oky[d] = by + d > ly && by + d < ny - 1;
+        okz[d] = bz + d > lz && bz + d < nz - 1;
+    }
+    const uint32_t base = (uint32_t)bx + dim[0] * ((uint32_t)by + dim[1] * (uint32_t)bz);
+    const uint32_t sy = dim[0], sz = dim[0] * dim[1];
     bool ok[8];
-    uint32_t idx[8];
     float fv[8], sw[8];
+    // The two x-taps of a (y, z) row are adjacent faces: one 8-byte load each
+    // from vel and sav.  A row is loaded when either of its taps is used;
+    // then both of its faces are in the grid (a used tap at bx + 1 means
+    // bx >= l >= 0, one at bx means bx + 1 <= n - 1).
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int32_t ii = bx + (t & 1), jj = by + ((t >> 1) & 1), kk = bz + (t >> 2);
-        // in range, and not *_face_touches_static_solid (sh/g2p.wgsl:35-54)
-        ok[t] = ii >= 0 && ii < (int32_t)dim[0] && jj >= 0 && jj < (int32_t)dim[1] && kk >= 0 && kk < (int32_t)dim[2] &&
-                !(ii <= lx || ii >= nx - 1 || jj <= ly || jj >= ny - 1 || kk <= lz || kk >= nz - 1);
-        idx[t] = ok[t] ? (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk) : 0u;
+    for (int rw = 0; rw < 4; ++rw) {
+        const int dj = rw & 1, dk = rw >> 1;
+        ok[2 * rw] = okx[0] && oky[dj] && okz[dk];
+        ok[2 * rw + 1] = okx[1] && oky[dj] && okz[dk];
+        const bool any = ok[2 * rw] || ok[2 * rw + 1];
+        // A mask rather than a select, so the offset stays straight-line code.
+        const uint32_t off = ((base + sy * (uint32_t)dj + sz * (uint32_t)dk) * 4u) & (0u - (uint32_t)any);
+        const f32x2_a4 a = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(vel) + off);
+        const f32x2_a4 b = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(sav) + off);
+        fv[2 * rw] = a.x;
+        fv[2 * rw + 1] = a.y;
+        sw[2 * rw] = b.x;
+        sw[2 * rw + 1] = b.y;
     }
+    const float wx[2] = {1.0f - tx, tx}, wy[2] = {1.0f - ty, ty}, wz[2] = {1.0f - tz, tz};
+    float wxy[4];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        fv[t] = vel[idx[t]];
-        sw[t] = sav[idx[t]];
-    }
+    for (int t = 0; t < 4; ++t) wxy[t] = wx[t & 1] * wy[t >> 1];
     float fin = 0.0f, sv = 0.0f, wsum = 0.0f;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-        const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
-        const float wz = dk == 1 ? tz : 1.0f - tz;
-        const float wy = dj == 1 ? ty : 1.0f - ty;
-        const float wx = di == 1 ? tx : 1.0f - tx;
-        const float w = wx * wy * wz;
+        const float w = wxy[t & 3] * wz[t >> 2];  // wx * wy * wz
         fin = ok[t] ? fin + w * fv[t] : fin;
         sv = ok[t] ? sv + w * sw[t] : sv;
         wsum = ok[t] ? wsum + w : wsum;
