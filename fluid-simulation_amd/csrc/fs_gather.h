@@ -24,7 +24,14 @@
 
 namespace fs {
 
-constexpr int kGThreads = 256;
+// Workgroup sizes of the gather (measured on MI355X).  One tank: 1024 threads
+// when the grid has at most kGSmallGrid bricks, so a brick's particles are
+// walked by more lanes at once (32^3, 64^3), else 256 (128^3 equal, 256^3
+// faster than 512).  The batch, with its many small tanks, uses 512.
+constexpr int kGThreads = 512;
+constexpr int kGThreadsSmall = 1024;
+constexpr int kGThreadsLarge = 256;
+constexpr uint32_t kGSmallGrid = 1024;
 
 // Brick shapes of the gather menu: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells
 // (0 measured fastest at 128^3, 256^3 and on the 64-tank batch).
@@ -41,7 +48,7 @@ struct GatherOut {
 // Brick at cell (x0, y0, z0).  sp: the sorted side; src: the side the keys
 // were computed on (listed indices refer to it); [pbeg, pend): the particle
 // range of this tank in both sides.
-template <int BF16, int V>
+template <int BF16, int V, int THREADS = kGThreads>
 __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const ParticleSoA& sp, const ParticleSoA& src,
                                                  const uint32_t* __restrict__ occ, const uint32_t* __restrict__ off,
                                                  const uint8_t* __restrict__ ct, const GatherOut& out, int32_t x0,
@@ -58,7 +65,7 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
     __shared__ uint32_t s_row_pref[kGRowsMax + 1];
     const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
 
-    for (int t = threadIdx.x; t < kGU + kGV + kGW; t += kGThreads) {
+    for (int t = threadIdx.x; t < kGU + kGV + kGW; t += THREADS) {
         s_num[t] = 0;
         s_den[t] = 0;
     }
@@ -145,11 +152,11 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
     };
 
     int row = 0;
-    for (uint32_t base = 0; base < total; base += kGThreads) {
-        // Each wave takes every 4th item of the chunk, so its lanes fall in
+    for (uint32_t base = 0; base < total; base += THREADS) {
+        // Each wave takes every (THREADS / 64)-th item of the chunk, so its lanes fall in
         // different cells (sorted particles of one cell share faces, and
         // same-address LDS atomics within a wave serialize).
-        const uint32_t t = base + (threadIdx.x & 63u) * (kGThreads / 64) + (threadIdx.x >> 6);
+        const uint32_t t = base + (threadIdx.x & 63u) * (THREADS / 64) + (threadIdx.x >> 6);
         if (t >= total) continue;
         // Find the row holding flattened item t (rows are few; linear advance).
         while (s_row_pref[row + 1] <= t) ++row;
@@ -158,7 +165,7 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
                              load_vel<BF16>(sp.vel[2], p)};
         add_particle(sp.pos[0][p], sp.pos[1][p], sp.pos[2][p], pv);
     }
-    for (uint32_t e = threadIdx.x; e < mis_n; e += kGThreads) {
+    for (uint32_t e = threadIdx.x; e < mis_n; e += THREADS) {
         const uint32_t p = mis_list[e];
         if (p < pbeg || p >= pend) continue;
         const float x = src.pos[0][p], y = src.pos[1][p], z = src.pos[2][p];
@@ -183,7 +190,7 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
         float* sav = out.saved[a];
         uint32_t dim[3];
         face_dims(P, a, dim);
-        for (int t = threadIdx.x; t < lxd * lyd * lzd; t += kGThreads) {
+        for (int t = threadIdx.x; t < lxd * lyd * lzd; t += THREADS) {
             const int32_t lx = t % lxd, ly = (t / lxd) % lyd, lz = t / (lxd * lyd);
             if (lx > hx || ly > hy || lz > hz) continue;
             const int32_t i = x0 + lx, j = y0 + ly, k = z0 + lz;

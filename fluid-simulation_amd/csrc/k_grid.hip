@@ -220,8 +220,8 @@ hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const ui
 // Sorted-segment P2G gather (fs_gather.h), one workgroup per brick.  When more
 // particles were listed as misfits than the gather walks, it exits and the
 // atomic form runs instead.
-template <int BF16, int V>
-__global__ __launch_bounds__(kGThreads) void k_p2g_gather(StepParams P, ParticleSoA sp, ParticleSoA src,
+template <int BF16, int V, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_p2g_gather(StepParams P, ParticleSoA sp, ParticleSoA src,
                                                            const uint32_t* __restrict__ occ,
                                                            const uint32_t* __restrict__ off,
                                                            const uint8_t* __restrict__ ct, GatherOut out,
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kGThreads) void k_p2g_gather(StepParams P, Particle
     const uint32_t G = nbx * nby * nbz;
     const uint32_t b = xcd_remap(blockIdx.x, G);
     const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
-    p2g_gather_brick<BF16, V>(P, sp, src, occ, off, ct, out, (int32_t)bxi * Brick<V>::X, (int32_t)byi * Brick<V>::Y,
+    p2g_gather_brick<BF16, V, THREADS>(P, sp, src, occ, off, ct, out, (int32_t)bxi * Brick<V>::X, (int32_t)byi * Brick<V>::Y,
                               (int32_t)bzi * Brick<V>::Z, mis_list, mis_n, 0u, P.N);
 }
 
@@ -245,12 +245,20 @@ hipError_t launch_gather_v(const StepParams& P, const ParticleSoA& sorted, const
     const uint32_t nbx = (P.nx + Brick<V>::X - 1) / Brick<V>::X, nby = (P.ny + Brick<V>::Y - 1) / Brick<V>::Y,
                    nbz = (P.nz + Brick<V>::Z - 1) / Brick<V>::Z;
     const uint32_t G = nbx * nby * nbz;
-    if (bf16)
-        hipLaunchKernelGGL((k_p2g_gather<1, V>), dim3(G), dim3(kGThreads), 0, s, P, sorted, src, occ, cell_offset, ct,
-                           out, nbx, nby, nbz, mis_count, mis_list);
-    else
-        hipLaunchKernelGGL((k_p2g_gather<0, V>), dim3(G), dim3(kGThreads), 0, s, P, sorted, src, occ, cell_offset, ct,
-                           out, nbx, nby, nbz, mis_count, mis_list);
+    if (G <= kGSmallGrid) {
+        if (bf16)
+            hipLaunchKernelGGL((k_p2g_gather<1, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
+                               occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+        else
+            hipLaunchKernelGGL((k_p2g_gather<0, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
+                               occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+    } else if (bf16) {
+        hipLaunchKernelGGL((k_p2g_gather<1, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
+                           occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+    } else {
+        hipLaunchKernelGGL((k_p2g_gather<0, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
+                           occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+    }
     return hipGetLastError();
 }
 }  // namespace
