@@ -18,6 +18,18 @@ __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Element i of an array as a 32-bit byte offset from a uniform base, so the
+// access is one VGPR offset on an SGPR base instead of a 64-bit address per
+// lane (arrays are below 4 GB: grids are limited to 2^30 faces).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T* base, uint32_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
 template <int THREADS>
 __device__ __forceinline__ double block_sum(double v, double* s_w) {
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
@@ -159,7 +171,7 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
     for (int u = 0; u < U; ++u) {
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
-        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+        m[u] = c[u] < C ? ld_at(nmask, c[u]) : (uint16_t)0;
     }
     // Every value load is issued without waiting for the mask (a neighbour
     // index is clamped into the grid); the mask selects the terms below, so a
@@ -168,14 +180,14 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const bool ok = c[u] < C;
-        rc[u] = (ok && !first) ? r[c[u]] : 0.0f;
-        dc[u] = ok ? dprev[c[u]] : 0.0f;
+        rc[u] = (ok && !first) ? ld_at(r, c[u]) : 0.0f;
+        dc[u] = ok ? ld_at(dprev, c[u]) : 0.0f;
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             int32_t nb = (int32_t)c[u] + off[s];
             nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
-            rn[u][s] = (ok && !first) ? r[nb] : 0.0f;
-            dnb[u][s] = ok ? dprev[nb] : 0.0f;
+            rn[u][s] = (ok && !first) ? ld_at(r, (uint32_t)nb) : 0.0f;
+            dnb[u][s] = ok ? ld_at(dprev, (uint32_t)nb) : 0.0f;
         }
     }
     double acc = 0.0;
@@ -191,8 +203,8 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
                 if ((m[u] >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
             qc = (float)(m[u] & 7u) * dn - sum;
         }
-        if (!first) dnext[c[u]] = dn;
-        q[c[u]] = qc;
+        if (!first) st_at(dnext, c[u], dn);
+        st_at(q, c[u], qc);
         acc += (double)dn * (double)qc;
     }
     return acc;
@@ -214,18 +226,18 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const bool ok = c[u] < C;
-        pv[u] = ok ? p[c[u]] : 0.0f;
-        dv[u] = ok ? dn[c[u]] : 0.0f;
-        rv[u] = ok ? r[c[u]] : 0.0f;
-        qv[u] = ok ? q[c[u]] : 0.0f;
+        pv[u] = ok ? ld_at(p, c[u]) : 0.0f;
+        dv[u] = ok ? ld_at(dn, c[u]) : 0.0f;
+        rv[u] = ok ? ld_at(r, c[u]) : 0.0f;
+        qv[u] = ok ? ld_at(q, c[u]) : 0.0f;
     }
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         if (c[u] >= C) continue;
-        p[c[u]] = pv[u] + alpha * dv[u];
+        st_at(p, c[u], pv[u] + alpha * dv[u]);
         const float rn = rv[u] - alpha * qv[u];
-        r[c[u]] = rn;
+        st_at(r, c[u], rn);
         acc += (double)rn * (double)rn;
     }
     return acc;
@@ -265,10 +277,10 @@ __device__ __forceinline__ double resident_load(const StepParams& P, const uint1
         const uint32_t t = tw.begin + (uint32_t)u * tw.stride;
         rc.c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
         const bool ok = rc.c[u] < C;
-        rc.m[u] = ok ? nmask[rc.c[u]] : (uint16_t)0;
-        rc.r[u] = ok ? r[rc.c[u]] : 0.0f;
-        rc.d[u] = ok ? d[rc.c[u]] : 0.0f;
-        rc.p[u] = ok ? p[rc.c[u]] : 0.0f;
+        rc.m[u] = ok ? ld_at(nmask, rc.c[u]) : (uint16_t)0;
+        rc.r[u] = ok ? ld_at(r, rc.c[u]) : 0.0f;
+        rc.d[u] = ok ? ld_at(d, rc.c[u]) : 0.0f;
+        rc.p[u] = ok ? ld_at(p, rc.c[u]) : 0.0f;
         rc.q[u] = 0.0f;
     }
 #pragma unroll
@@ -296,8 +308,8 @@ __device__ __forceinline__ double resident_phase_a(const StepParams& P, const fl
         for (int s = 0; s < 6; ++s) {
             int32_t nb = (int32_t)cu + off[s];
             nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
-            rn[u][s] = (ok && !first) ? r[nb] : 0.0f;
-            dnb[u][s] = ok ? dprev[nb] : 0.0f;
+            rn[u][s] = (ok && !first) ? ld_at(r, (uint32_t)nb) : 0.0f;
+            dnb[u][s] = ok ? ld_at(dprev, (uint32_t)nb) : 0.0f;
         }
     }
     double acc = 0.0;
@@ -373,7 +385,7 @@ __device__ __forceinline__ void cgs_init_batch(const StepParams& P, const uint16
     for (int u = 0; u < U; ++u) {
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
-        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+        m[u] = c[u] < C ? ld_at(nmask, c[u]) : (uint16_t)0;
         const bool ok = c[u] < C;
         rc[u] = ok ? r[c[u]] : 0.0f;
 #pragma unroll
@@ -420,7 +432,7 @@ __device__ __forceinline__ void cgs_sweep_batch(const StepParams& P, const uint1
     for (int u = 0; u < U; ++u) {
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
-        m[u] = c[u] < C ? nmask[c[u]] : (uint16_t)0;
+        m[u] = c[u] < C ? ld_at(nmask, c[u]) : (uint16_t)0;
         const bool ok = c[u] < C;
         rc[u] = ok ? r[c[u]] : 0.0f;
         wc[u] = ok ? w[c[u]] : 0.0f;
@@ -473,7 +485,7 @@ __device__ __forceinline__ double rr_batch(const StepParams& P, const float* __r
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : P.C;
         if (c < P.C) {
-            const double rv = (double)r[c];
+            const double rv = (double)ld_at(r, c);
             acc += rv * rv;
         }
     }
@@ -492,7 +504,7 @@ __device__ __forceinline__ double true_res_batch(const StepParams& P, const uint
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : C;
         if (c >= C) continue;
-        const uint16_t m = nmask[c];
+        const uint16_t m = ld_at(nmask, c);
         if (!(m & kMaskLiquid)) continue;
         double sum = 0.0;
 #pragma unroll

@@ -187,8 +187,9 @@ bool all_finite(const float* a, size_t n) {
 // Creation-time checks shared by fs_create and fs_batch_create.
 int check_params(const fs_params& q, uint32_t n, const float* pos_xyz, const float* vel_xyz) {
     if (q.nx < 3 || q.ny < 3 || q.nz < 3) return fail(FS_ERR_INVALID_ARGUMENT, "grid must be at least 3 cells per axis");
-    if ((uint64_t)(q.nx + 1) * (q.ny + 1) * (q.nz + 1) >= (1ull << 31))
-        return fail(FS_ERR_INVALID_ARGUMENT, "grid too large for 32-bit face indices");
+    // Kernels address cell and face arrays with 32-bit byte offsets.
+    if ((uint64_t)(q.nx + 1) * (q.ny + 1) * (q.nz + 1) >= (1ull << 30))
+        return fail(FS_ERR_INVALID_ARGUMENT, "grid too large for 32-bit byte offsets");
     if (!(q.h > 0.0f) || !std::isfinite(q.h) || !(q.dt > 0.0f) || !std::isfinite(q.dt) || !(q.rho > 0.0f))
         return fail(FS_ERR_INVALID_ARGUMENT, "h, dt and rho must be positive and finite");
     const float scalars[] = {q.origin[0], q.origin[1], q.origin[2], q.gravity[0], q.gravity[1], q.gravity[2],
