@@ -210,14 +210,20 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
     return acc;
 }
 
-template <int U, typename TileId>
+// Phase B: r -= alpha q and r.r, and the p update selected by PMODE:
+// kPOne p += alpha dn; kPNone p left alone; kPTwo p = (p + alpha_prev dprev)
+// + alpha dn, the deferred update of iteration k-1 and the one of iteration
+// k in their own order (the same f32 operations as one update per
+// iteration, so deferring every other update changes no bit).
+constexpr int kPNone = 0, kPOne = 1, kPTwo = 2;
+template <int U, int PMODE, typename TileId>
 __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float* __restrict__ dn,
                                                 const float* __restrict__ q, float* __restrict__ p,
-                                                float* __restrict__ r, float alpha, const TileWalk& tw, uint32_t ti,
-                                                TileId&& tile_id) {
+                                                float* __restrict__ r, float alpha, const float* __restrict__ dprev,
+                                                float alpha_prev, const TileWalk& tw, uint32_t ti, TileId&& tile_id) {
     const uint32_t lane = lane_id(), C = P.C;
     uint32_t c[U];
-    float pv[U], dv[U], rv[U], qv[U];
+    float pv[U], dv[U], dpv[U], rv[U], qv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t t = ti + (uint32_t)u * tw.stride;
@@ -226,8 +232,11 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const bool ok = c[u] < C;
-        pv[u] = ok ? ld_at(p, c[u]) : 0.0f;
-        dv[u] = ok ? ld_at(dn, c[u]) : 0.0f;
+        if (PMODE != kPNone) {
+            pv[u] = ok ? ld_at(p, c[u]) : 0.0f;
+            dv[u] = ok ? ld_at(dn, c[u]) : 0.0f;
+        }
+        if (PMODE == kPTwo) dpv[u] = ok ? ld_at(dprev, c[u]) : 0.0f;
         rv[u] = ok ? ld_at(r, c[u]) : 0.0f;
         qv[u] = ok ? ld_at(q, c[u]) : 0.0f;
     }
@@ -235,12 +244,28 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         if (c[u] >= C) continue;
-        st_at(p, c[u], pv[u] + alpha * dv[u]);
+        if (PMODE == kPOne) st_at(p, c[u], pv[u] + alpha * dv[u]);
+        if (PMODE == kPTwo) st_at(p, c[u], (pv[u] + alpha_prev * dpv[u]) + alpha * dv[u]);
         const float rn = rv[u] - alpha * qv[u];
         st_at(r, c[u], rn);
         acc += (double)rn * (double)rn;
     }
     return acc;
+}
+
+// The last pending p update (p += alpha dn) after an iteration whose update
+// was deferred.
+template <int U, typename TileId>
+__device__ __forceinline__ void p_update_batch(const StepParams& P, const float* __restrict__ dn,
+                                               float* __restrict__ p, float alpha, const TileWalk& tw, uint32_t ti,
+                                               TileId&& tile_id) {
+    const uint32_t lane = lane_id(), C = P.C;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t t = ti + (uint32_t)u * tw.stride;
+        const uint32_t c = t < tw.end ? tile_id(t) * kTile + lane : C;
+        if (c < C) st_at(p, c, ld_at(p, c) + alpha * ld_at(dn, c));
+    }
 }
 
 // ---- register-resident form of the two phases ----

@@ -450,6 +450,11 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
     float* dnext = d1 + co;
     uint32_t iters = 0;
     bool active = has_tank;
+    // p takes the updates of iterations k-1 and k together at odd k
+    // (phase_b_batch kPTwo); a tank stopping after an even k applies the
+    // pending one then.  A tank's iteration index k is `it` while it is
+    // active (all tanks start together).
+    float alpha_prev = 0.0f;
     for (uint32_t it = 0;; ++it) {
         const bool first = it == 0;
         float* dn_buf = first ? dprev : dnext;
@@ -467,9 +472,12 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
         if (!any) break;  // every tank has finished (the same flags are seen by all)
         const float alpha = fabs(dq) > 1e-30 ? (float)(rs_old / dq) : 0.0f;
         acc = 0.0;
-        if (active)
+        if (active && (it & 1u))
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_b_batch<U>(P, dn_buf, q_, pp, rr_, alpha, tw, ti, tile_id);
+                acc += phase_b_batch<U, kPTwo>(P, dn_buf, q_, pp, rr_, alpha, dprev, alpha_prev, tw, ti, tile_id);
+        else if (active)
+            for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                acc += phase_b_batch<U, kPNone>(P, dn_buf, q_, pp, rr_, alpha, nullptr, 0.0f, tw, ti, tile_id);
         double rr;
         if (!lockstep_sum<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, rr, any)) {
             report(rs_init, rs_old, 0.0, iters, 1u);
@@ -478,14 +486,25 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
         if (active) {
             beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
             rs_old = rr;
+            alpha_prev = alpha;
             if ((tol > 0.0 && rr <= tol_sq * rs_init) || iters >= P.iters) {
                 active = false;
+                if (!(it & 1u))
+                    for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                        p_update_batch<U>(P, dn_buf, pp, alpha, tw, ti, tile_id);
             } else if (!first) {
                 float* t = dprev;
                 dprev = dnext;
                 dnext = t;
             }
         }
+    }
+    // A pending p update of a stopping tank comes after its last barrier:
+    // one more barrier publishes p before the true residual reads neighbours.
+    double unused;
+    if (!lockstep_sum<kBCgThreads>(0.0, 0u, w, slot, base, nb, unused, any)) {
+        report(rs_init, rs_old, 0.0, iters, 1u);
+        return;
     }
     acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<U>(P, nm, bb, pp, tw, ti, tile_id);

@@ -288,6 +288,10 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
     float beta = 0.0f;
     float* dprev = d0;
     float* dnext = d1;
+    // Streaming form: p takes the updates of iterations k-1 and k together at
+    // odd k (phase_b_batch kPTwo); an even last iteration leaves one pending.
+    float alpha_prev = 0.0f, last_alpha = 0.0f;
+    const float* last_dn = d0;
     iters = 0;
     for (uint32_t it = 0; it < P.iters; ++it) {
         ++iters;
@@ -306,9 +310,15 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
         acc = 0.0;
         if (RES)
             acc = resident_phase_b<kU>(P, r, alpha, rc);
+        else if (it & 1u)
+            for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
+                acc += phase_b_batch<kU, kPTwo>(P, dn_buf, q, p, r, alpha, dprev, alpha_prev, tw, ti, tile_id);
         else
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_b_batch<kU>(P, dn_buf, q, p, r, alpha, tw, ti, tile_id);
+                acc += phase_b_batch<kU, kPNone>(P, dn_buf, q, p, r, alpha, nullptr, 0.0f, tw, ti, tile_id);
+        alpha_prev = alpha;
+        last_alpha = alpha;
+        last_dn = dn_buf;
         double rr;
         if (!grid_sum<THREADS>(acc, w, G, rr)) return false;
         beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
@@ -320,10 +330,13 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
             dnext = t;
         }
     }
-    if (RES)
+    if (RES) {
 #pragma unroll
         for (int u = 0; u < kU; ++u)
             if (rc.c[u] < P.C) p[rc.c[u]] = rc.p[u];
+    } else if (iters & 1u) {
+        for (uint32_t ti = tw.begin; ti < tw.end; ti += step) p_update_batch<kU>(P, last_dn, p, last_alpha, tw, ti, tile_id);
+    }
     rs_init_out = rs_init;
     rs_final = rs_old;
     return true;
@@ -368,8 +381,9 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
-    // Resident blocks stored p after the loop: one more barrier publishes it
-    // before the true residual reads neighbours' p.
+    // Resident blocks store p after the loop and streaming ones may finish a
+    // pending update there: one more barrier publishes p before the true
+    // residual reads neighbours' p.
     double unused;
     if (!grid_sum<THREADS>(0.0, w, G, unused)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
@@ -591,7 +605,7 @@ __global__ __launch_bounds__(kCgMultiThreads) void k_cg_phase_b(StepParams P, co
     auto tile_id = [&](uint32_t t) -> uint32_t { return gt[t]; };
     double acc = 0.0;
     for (uint32_t ti = tw.begin; ti < tw.end; ti += tw.stride * kU)
-        acc += phase_b_batch<kU>(P, dn, q, p, r, alpha, tw, ti, tile_id);
+        acc += phase_b_batch<kU, kPOne>(P, dn, q, p, r, alpha, nullptr, 0.0f, tw, ti, tile_id);
     double rr;
     if (last_block_sum<kCgMultiThreads>(acc, w, rr) && threadIdx.x == 0) {
         const double rs_old = st->rs_old;

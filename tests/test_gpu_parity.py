@@ -300,3 +300,27 @@ def test_full_size_high_density_one_step():
     assert np.all(np.isfinite(gp)) and np.all(np.isfinite(gv))
     assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
     t.close()
+
+
+@pytest.mark.parametrize("cap", [30, 31])
+def test_streaming_solve_parity(cap, monkeypatch):
+    """The persistent solve's streaming form (a workgroup walks its tiles in
+    batches instead of holding them in registers; p takes two iterations'
+    updates at a time).  A launch capped at 8 workgroups gives every
+    workgroup of this 64^3 tank more tiles than its waves hold.  With the
+    tolerance off the solve runs exactly `cap` iterations; an odd count leaves
+    one deferred p update for after the loop."""
+    monkeypatch.setenv("FLUIDSTEP_CG_BLOCKS", "8")
+    sc = scenes.control_64()
+    say("streaming solve parity", sc["name"], "cap", cap)
+    ref, t = run_pair(sc, fsm.ENGINE_FUSED, pressure_iterations=cap, pressure_residual_tolerance=0.0)
+    st = t.stats()
+    assert st["cg_iterations"] == cap and ref.stats.cg_iterations == cap
+    assert rel_l2(t.pressure(), ref.pressure) <= TOL
+    vel = t.faces(fsm.FACE_VELOCITY)
+    for a in range(3):
+        assert rel_l2(vel[a], ref.vel_final[a]) <= TOL
+    pos, pv = t.particles()
+    assert rel_l2(pos, ref.pos) <= TOL
+    assert rel_l2(pv, ref.vel_particles) <= TOL
+    t.close()
