@@ -111,7 +111,7 @@ struct CgState {
     uint32_t pad;
 };
 struct CgWork {
-    // grid-barrier + reduction scratch; every word below up to `slots`
+    // grid-barrier + reduction scratch; every word below up to `slots_data`
     // (inclusive) lives in one area of cg_sync_words(max_blocks) u32 words
     // that launch_cg_reset clears before each solve
     uint32_t* sync;          // base of the area
@@ -120,7 +120,8 @@ struct CgWork {
     uint32_t* group_count;   // [8 * 32]: one arrival counter per 128-byte line
     uint32_t* bar_gen;       // [1]
     uint32_t* abort_flag;    // [1]
-    uint64_t* slots;         // [2][max_blocks][2] epoch-tagged partials
+    uint64_t* slots;         // [2][max_blocks][2] epoch-tagged partials (exchanges)
+    uint64_t* slots_data;    // [2][max_blocks][2] epoch-tagged partials (tagged data barriers)
     uint16_t* nmask;         // [C] stencil mask of liquid cells
     double* partials;        // [max_blocks]
     double* result;          // [4]

@@ -36,7 +36,7 @@ constexpr size_t kSyncBarCount = 0, kSyncGroups = 32, kSyncAbort = 32 + 8 * 32, 
                  kSyncSlots = kSyncGen + 32;
 }  // namespace
 
-size_t cg_sync_words(int max_blocks) { return kSyncSlots + 8u * (size_t)(max_blocks > 0 ? max_blocks : 1); }
+size_t cg_sync_words(int max_blocks) { return kSyncSlots + 16u * (size_t)(max_blocks > 0 ? max_blocks : 1); }
 
 void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks) {
     w.sync = area;
@@ -46,6 +46,7 @@ void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks) {
     w.abort_flag = area + kSyncAbort;
     w.bar_gen = area + kSyncGen;
     w.slots = reinterpret_cast<uint64_t*>(area + kSyncSlots);
+    w.slots_data = reinterpret_cast<uint64_t*>(area + kSyncSlots + 8u * (size_t)(max_blocks > 0 ? max_blocks : 1));
 }
 
 hipError_t launch_cg_reset(const CgWork& w, hipStream_t s) { return launch_fill_u32(w.sync, 0u, w.sync_words, s); }
@@ -229,6 +230,52 @@ __device__ bool grid_exchange(double v, const CgWork& w, uint32_t G, uint32_t ep
     return true;
 }
 
+// Grid barrier + sum without counters: every block's stores before it are
+// visible to every block after it.  Thread 0 issues a release fence (writes
+// the XCD's L2 back) and publishes the block's partial in its epoch-tagged
+// slot; wave 0 reads all G slots (lane l takes slots l, l+64, ...) until each
+// carries the epoch, then issues the acquire fence in those same lanes (each
+// lane's fence follows its own reads of the tags it waited for), and the
+// workgroup barrier orders the other waves after it.  The total is lane 0's
+// butterfly over the lanes' in-order sums: one fixed order, so every block
+// holds the identical total.  Slots alternate by epoch parity as in
+// grid_exchange; this barrier has its own slot area and epoch sequence.
+template <int THREADS>
+__device__ bool grid_sum_tagged(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
+    __shared__ double s_out;
+    __shared__ int s_abort;
+    uint64_t* half = w.slots_data + (size_t)(epoch & 1u) * G * 2u;
+    const double in[1] = {v};
+    double bs[1];
+    block_sum_nv<THREADS, 1>(in, bs);
+    if (threadIdx.x == 0) {
+        s_abort = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        publish_tagged(half, blockIdx.x, epoch, bs[0], 0u);
+    }
+    if (threadIdx.x < 64) {
+        double acc = 0.0;
+        bool ok = true;
+        for (uint32_t i = threadIdx.x; i < G; i += 64) {
+            double x;
+            uint32_t f;
+            if (!poll_tagged(half, i, epoch, w.abort_flag, x, f)) {
+                ok = false;
+                break;
+            }
+            acc += x;
+        }
+        if (!ok) s_abort = 1;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const double t = wave_sum(acc);
+        if (threadIdx.x == 0) s_out = t;
+    }
+    __syncthreads();
+    if (s_abort) return false;
+    out = s_out;
+    return true;
+}
+
 // Last-block-done reduction: no block waits; the last to arrive gets the
 // total (valid in thread 0) and returns true.
 template <int THREADS>
@@ -262,17 +309,34 @@ __device__ __forceinline__ uint32_t solve_blocks(uint32_t nt, uint32_t per_block
     return g < launched ? g : launched;
 }
 
+// Form of the reductions that order memory: kDbCounter (grid_sum),
+// kDbTagged (grid_sum_tagged) or kDbAuto: tagged slots up to kTaggedMaxBlocks
+// participating workgroups, counters above.  Measured per iteration on
+// MI355X: 8 workgroups 7.2 vs 9.5 us, 16 about 9.3 vs 11.0, 208 equal, 256
+// (streaming 256^3) 51.0 vs 50.4.
+constexpr int kDbCounter = 0, kDbTagged = 1, kDbAuto = 2;
+constexpr uint32_t kTaggedMaxBlocks = 128;
+template <int DB>
+__device__ __forceinline__ bool use_tagged(uint32_t G) {
+    return DB == kDbTagged || (DB == kDbAuto && G <= kTaggedMaxBlocks);
+}
+
 // The iteration loop of the two-phase solve over one block's tiles; RES
 // keeps the cells in registers (every wave has at most one batch of kU
 // tiles).  Returns false on abort.  Both forms run the same reductions in
 // the same order and the same f32 expressions.
-template <int THREADS, bool EXCH, bool RES, typename TileId>
+template <int THREADS, bool EXCH, int DB, bool RES, typename TileId>
 __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* __restrict__ b, float* __restrict__ p,
                                              float* __restrict__ r, float* __restrict__ d0, float* __restrict__ d1,
                                              float* __restrict__ q, const CgWork& w, uint32_t G, const TileWalk& tw,
                                              TileId&& tile_id, double& rs_init_out, double& rs_final,
                                              uint32_t& iters) {
     const uint32_t step = tw.stride * kU;
+    // The reductions that order memory: counters (grid_sum) or tagged slots
+    // (grid_sum_tagged, epochs 1 = r.r of r_0, k + 2 = iteration k).
+    auto data_sum = [&](double v, uint32_t epoch, double& out) -> bool {
+        return use_tagged<DB>(G) ? grid_sum_tagged<THREADS>(v, w, G, epoch, out) : grid_sum<THREADS>(v, w, G, out);
+    };
     ResidentCells<kU> rc;
     double acc = 0.0;
     if (RES)
@@ -280,7 +344,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
     else
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += rr_batch<kU>(P, r, tw, ti, tile_id);
     double rs0;
-    if (!grid_sum<THREADS>(acc, w, G, rs0)) return false;
+    if (!data_sum(acc, 1u, rs0)) return false;
     double rs_old = rs0;
     const double rs_init = rs0;
     const double tol = (double)P.tol;
@@ -320,7 +384,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
         last_alpha = alpha;
         last_dn = dn_buf;
         double rr;
-        if (!grid_sum<THREADS>(acc, w, G, rr)) return false;
+        if (!data_sum(acc, it + 2u, rr)) return false;
         beta = rs_old > 1e-30 ? (float)(rr / rs_old) : 0.0f;
         rs_old = rr;
         if (tol > 0.0 && rr <= tol_sq * rs_init) break;
@@ -343,8 +407,9 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
 }
 
 // THREADS: workgroup size; EXCH: the reduction after phase A (dn.q, which
-// phase B consumes with data only its own lanes wrote) uses grid_exchange.
-template <int THREADS, bool EXCH>
+// phase B consumes with data only its own lanes wrote) uses grid_exchange;
+// DB: the form of the reductions that order memory (use_tagged).
+template <int THREADS, bool EXCH, int DB>
 __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const uint8_t* __restrict__ ct,
                                                                const float* __restrict__ b, float* __restrict__ p,
                                                                float* __restrict__ r, float* __restrict__ d0,
@@ -373,10 +438,10 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     double rs_init = 0.0, rs_old = 0.0;
     uint32_t iters = 0;
     const bool ok = resident
-                        ? cg_two_phase<THREADS, EXCH, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
-                                                            iters)
-                        : cg_two_phase<THREADS, EXCH, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
-                                                             rs_old, iters);
+                        ? cg_two_phase<THREADS, EXCH, DB, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
+                                                                    rs_old, iters)
+                        : cg_two_phase<THREADS, EXCH, DB, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
+                                                                     rs_old, iters);
     if (!ok) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
@@ -385,7 +450,9 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     // pending update there: one more barrier publishes p before the true
     // residual reads neighbours' p.
     double unused;
-    if (!grid_sum<THREADS>(0.0, w, G, unused)) {
+    const bool okp = use_tagged<DB>(G) ? grid_sum_tagged<THREADS>(0.0, w, G, iters + 2u, unused)
+                                       : grid_sum<THREADS>(0.0, w, G, unused);
+    if (!okp) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
     }
@@ -498,25 +565,38 @@ namespace {
 // Persistent-kernel variant: FLUIDSTEP_CG_FORM=twophase (default) or
 // single; workgroup size 1024 (default) or 512 (FLUIDSTEP_CG_THREADS);
 // for the two-phase form FLUIDSTEP_CG_EXCHANGE=0 keeps grid_sum after
-// phase A.
+// phase A, and FLUIDSTEP_CG_DATA_BARRIER=counter|tagged fixes the form of
+// the memory-ordering reductions (default: by workgroup count, use_tagged).
 struct PersistentVariant {
     int threads;
     bool single;
     bool exch;
+    int db;
     const void* fn;
 };
+
+template <int T, bool X>
+const void* two_phase_fn_x(int db) {
+    if (db == kDbCounter) return (const void*)k_cg_persistent<T, X, kDbCounter>;
+    if (db == kDbTagged) return (const void*)k_cg_persistent<T, X, kDbTagged>;
+    return (const void*)k_cg_persistent<T, X, kDbAuto>;
+}
+template <int T>
+const void* two_phase_fn(bool exch, int db) {
+    return exch ? two_phase_fn_x<T, true>(db) : two_phase_fn_x<T, false>(db);
+}
 PersistentVariant persistent_variant() {
     static const PersistentVariant v = [] {
-        PersistentVariant r{1024, false, true, nullptr};
+        PersistentVariant r{1024, false, true, kDbAuto, nullptr};
         if (const char* e = std::getenv("FLUIDSTEP_CG_THREADS")) r.threads = std::atoi(e) == 512 ? 512 : 1024;
         if (const char* e = std::getenv("FLUIDSTEP_CG_FORM")) r.single = std::strcmp(e, "single") == 0;
         if (const char* e = std::getenv("FLUIDSTEP_CG_EXCHANGE")) r.exch = std::atoi(e) != 0;
+        if (const char* e = std::getenv("FLUIDSTEP_CG_DATA_BARRIER"))
+            r.db = std::strcmp(e, "tagged") == 0 ? kDbTagged : (std::strcmp(e, "counter") == 0 ? kDbCounter : kDbAuto);
         if (r.single)
             r.fn = r.threads == 1024 ? (const void*)k_cg_single<1024, 2> : (const void*)k_cg_single<512, 2>;
-        else if (r.threads == 1024)
-            r.fn = r.exch ? (const void*)k_cg_persistent<1024, true> : (const void*)k_cg_persistent<1024, false>;
         else
-            r.fn = r.exch ? (const void*)k_cg_persistent<512, true> : (const void*)k_cg_persistent<512, false>;
+            r.fn = r.threads == 1024 ? two_phase_fn<1024>(r.exch, r.db) : two_phase_fn<512>(r.exch, r.db);
         return r;
     }();
     return v;
