@@ -135,17 +135,21 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
             }
             if (!((ox[0] || ox[1]) && (oy[0] || oy[1]) && (oz[0] || oz[1]))) continue;
             const float vcomp = pv[a];
+            const float wx[2] = {1.0f - tx, tx}, wy[2] = {1.0f - ty, ty}, wz[2] = {1.0f - tz, tz};
+            float wxy[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wxy[q] = wx[q & 1] * wy[q >> 1];
+            const int32_t li0 = base + (bx - x0) + lx_dim * ((by - y0) + ly_dim * (bz - z0));
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
-                const float wz = dk == 1 ? tz : 1.0f - tz;
-                const float wy = dj == 1 ? ty : 1.0f - ty;
-                const float wx = di == 1 ? tx : 1.0f - tx;
-                const float w = wx * wy * wz;
+                const float w = wxy[t & 3] * wz[dk];  // wx * wy * wz
                 if (ox[di] && oy[dj] && oz[dk] && w > 0.0f) {
-                    const int32_t li = base + (bx + di - x0) + lx_dim * ((by + dj - y0) + ly_dim * (bz + dk - z0));
+                    const int32_t li = li0 + di + lx_dim * (dj + ly_dim * dk);
                     atomicAdd(&s_num[li], sat_i32(rintf(w * vcomp * P.scale)));
-                    atomicAdd(&s_den[li], sat_i32(rintf(w * P.scale)));
+                    // w is in (0, 1] and P.scale is the fixed 2^16, so the weight
+                    // term cannot saturate: the plain conversion equals sat_i32.
+                    atomicAdd(&s_den[li], (int32_t)rintf(w * P.scale));
                 }
             }
         }
