@@ -20,6 +20,7 @@
 // owned taps; particles inside the halo were already counted by the segment
 // walk.  Each (particle, face) term is therefore added exactly once.
 #pragma once
+#include <type_traits>
 #include "fs_particle.h"
 
 namespace fs {
@@ -140,18 +141,29 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
 #pragma unroll
             for (int q = 0; q < 4; ++q) wxy[q] = wx[q & 1] * wy[q >> 1];
             const int32_t li0 = base + (bx - x0) + lx_dim * ((by - y0) + ly_dim * (bz - z0));
+            // Conversions to fixed point.  w is in (0, 1] and P.scale is the
+            // fixed 2^16, so the weight term never saturates; the velocity
+            // term cannot either when |v| * scale < 2e9 (|fl(w v)| <= |v|),
+            // which is false for NaN.  Where a conversion cannot saturate the
+            // plain one equals sat_i32, so only lanes with extreme or NaN
+            // velocities take the saturating loop.
+            auto add_taps = [&](auto plain_v) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
-                const float w = wxy[t & 3] * wz[dk];  // wx * wy * wz
-                if (ox[di] && oy[dj] && oz[dk] && w > 0.0f) {
-                    const int32_t li = li0 + di + lx_dim * (dj + ly_dim * dk);
-                    atomicAdd(&s_num[li], sat_i32(rintf(w * vcomp * P.scale)));
-                    // w is in (0, 1] and P.scale is the fixed 2^16, so the weight
-                    // term cannot saturate: the plain conversion equals sat_i32.
-                    atomicAdd(&s_den[li], (int32_t)rintf(w * P.scale));
+                for (int t = 0; t < 8; ++t) {
+                    const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
+                    const float w = wxy[t & 3] * wz[dk];  // wx * wy * wz
+                    if (ox[di] && oy[dj] && oz[dk] && w > 0.0f) {
+                        const int32_t li = li0 + di + lx_dim * (dj + ly_dim * dk);
+                        const float tv = rintf(w * vcomp * P.scale);
+                        atomicAdd(&s_num[li], decltype(plain_v)::value ? (int32_t)tv : sat_i32(tv));
+                        atomicAdd(&s_den[li], (int32_t)rintf(w * P.scale));
+                    }
                 }
-            }
+            };
+            if (fabsf(vcomp) * P.scale < 2.0e9f)
+                add_taps(std::true_type{});
+            else
+                add_taps(std::false_type{});
         }
     };
 

@@ -324,3 +324,23 @@ def test_streaming_solve_parity(cap, monkeypatch):
     assert rel_l2(pos, ref.pos) <= TOL
     assert rel_l2(pv, ref.vel_particles) <= TOL
     t.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_saturating_p2g_velocities(engine):
+    """A few particles fast enough that w * v * 2^16 leaves the i32 range:
+    the P2G conversion saturates as WGSL's i32() does, on the gather's
+    saturating path (the plain conversion is taken only where it cannot
+    saturate).  The integer sums, hence `saved`, stay bit-exact."""
+    sc = scenes.small_tank(n=16, seed=11)
+    vel = np.array(sc["vel"], np.float32, copy=True)
+    fast = np.arange(0, len(vel), max(1, len(vel) // 7))[:7]
+    vel[fast] = np.array([[2.0e5, -3.0e5, 1.0e9]] * len(fast), np.float32)
+    vel[fast[1::2], 0] = -4.0e4
+    say("saturating P2G", engine, "fast particles", len(fast))
+    ref, t = run_pair(sc, engine, vel=vel)
+    assert np.array_equal(t.cell_type(), ref.cell_type)
+    saved = t.faces(fsm.FACE_SAVED)
+    for a in range(3):
+        assert bits_equal(saved[a], ref.saved[a]), f"saved axis {a}"
+    t.close()
