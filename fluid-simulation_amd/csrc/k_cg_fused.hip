@@ -6,9 +6,17 @@
 //    are liquid) and flags the 64-cell tiles that hold liquid.
 //  * the liquid tiles are compacted into a list (exclusive scan), so the solve
 //    never touches all-air or all-solid tiles.
-//  * the solve itself, in one of two forms:
-//      - k_cg_persistent: one launch for the whole solve, two grid-wide
-//        reductions per iteration;
+//  * the solve itself, in one of three forms:
+//      - k_cg_persistent (the default): one cooperative launch for the whole
+//        solve on the fewest workgroups that keep every wave's tiles in one
+//        batch (solve_blocks).  A workgroup keeps its cells in registers when
+//        they fit (ResidentCells) and streams them otherwise.  Two grid-wide
+//        reductions per iteration: a counter-free exchange after phase A
+//        (grid_exchange) and a memory-ordering one after phase B (grid_sum
+//        or grid_sum_tagged, by workgroup count);
+//      - k_cg_single: the same CG in the single-reduction arrangement (one
+//        sweep and one barrier per iteration), kept as a measured
+//        alternative: its rounding path misses the per-step parity bar;
 //      - k_cg_phase_a / k_cg_phase_b: two launches per iteration, each ending
 //        in a last-block-done reduction (no block ever waits), enqueued for
 //        the iteration cap and replayed from a hipGraph by the host.
@@ -101,8 +109,9 @@ hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uin
 // Each block sums its waves in wave order and publishes one partial.  Arrival
 // is counted in two levels (a counter per group of blocks with equal
 // blockIdx % 8, then one counter over the groups), so no single address takes
-// more than G/8 atomics.  The block arriving last sums the partials in block
-// order (a fixed tree), so the total does not depend on timing.  Ordering:
+// more than G/8 atomics.  The partials are summed in one fixed order (by the
+// block arriving last in last_block_sum, by every block in grid_sum), so the
+// total does not depend on timing.  Ordering:
 // every wave's stores reach L2 before the workgroup barrier; thread 0's
 // release on its counter writes the XCD's L2 back; the consumer's acquire
 // invalidates its caches once.  Waiting blocks poll with relaxed loads (an
