@@ -134,7 +134,9 @@ FS_API int fs_registry_to_params(const fs_registry* r, float h, const float orig
 FS_API int fs_params_default(fs_params* p);
 /* GpuFluid::new (src/gpu/fluid.rs:247-1197): allocate every buffer and upload
  * the initial particles.  pos_xyz / vel_xyz hold n xyz triplets (vel may be
- * NULL = zero).  Non-finite inputs are rejected. */
+ * NULL = zero).  Non-finite inputs are rejected, and so are grids of 2^30 or
+ * more faces ((nx+1)(ny+1)(nz+1)): kernels address their arrays with 32-bit
+ * byte offsets. */
 FS_API int fs_create(const fs_params* p, const float* pos_xyz, const float* vel_xyz, uint32_t n, fs_tank** out);
 FS_API int fs_destroy(fs_tank* t);
 /* GpuFluid::reset (src/gpu/fluid.rs:1408-1418): initial particles back,
