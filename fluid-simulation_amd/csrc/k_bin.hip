@@ -232,18 +232,32 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
 // Each wave ranks 64*ITEMS consecutive keys in ITEMS rounds of 64; equal
 // digits inside a round are found with eight ballots.  Order of items =
 // (wave, round, lane) = input order, so equal digits keep their input order.
+// Output position of an item = start of its digit's run for this tile (from
+// the scanned histogram) + the counts of its digit in earlier waves + its
+// rank.  Tiles of up to 4096 keys are staged in LDS in that order, digit by
+// digit, and written out as contiguous runs per digit (coalesced stores);
+// larger tiles write each item directly.  The positions are the same either
+// way.
 template <int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
     const uint32_t* __restrict__ offsets, uint32_t n, uint32_t shift, uint32_t ntiles,
     int write_keys) {
+    constexpr bool kStage = ITEMS <= 16;
+    constexpr int kTileKeys = ITEMS * kBlock;
+    static_assert(kBlock == kRadixBins, "one thread per digit");
     __shared__ uint32_t s_cnt[kRadixWaves][kRadixBins];
+    __shared__ uint32_t s_lbase[kRadixBins];  // tile-local start of each digit's run
+    __shared__ uint32_t s_gbase[kRadixBins];  // global start of each digit's run of this tile
+    __shared__ uint32_t s_wtot[kRadixWaves];
+    __shared__ uint32_t s_k[kStage ? kTileKeys : 1], s_v[kStage ? kTileKeys : 1];
     for (int i = threadIdx.x; i < kRadixWaves * kRadixBins; i += kBlock) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    const size_t base = (size_t)blockIdx.x * (ITEMS * kBlock) + (size_t)w * (ITEMS * 64);
+    const size_t tile_base = (size_t)blockIdx.x * kTileKeys;
+    const size_t base = tile_base + (size_t)w * (ITEMS * 64);
     uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
@@ -270,22 +284,55 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     }
     __syncthreads();
     {
-        // digit t: exclusive offsets over waves + global base of (digit, tile).
+        // Thread t owns digit t: prefix of its counts over the waves, the
+        // global start of its run, and (block scan) its tile-local start.
         const uint32_t t = threadIdx.x;
-        uint32_t run = offsets[(size_t)t * ntiles + blockIdx.x];
+        uint32_t tot = 0;
         for (int ww = 0; ww < kRadixWaves; ++ww) {
-            uint32_t c = s_cnt[ww][t];
-            s_cnt[ww][t] = run;
-            run += c;
+            const uint32_t c = s_cnt[ww][t];
+            s_cnt[ww][t] = tot;
+            tot += c;
         }
+        s_gbase[t] = offsets[(size_t)t * ntiles + blockIdx.x];
+        uint32_t inc = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t up = __shfl_up(inc, o, 64);
+            if ((int)lane >= o) inc += up;
+        }
+        if (lane == 63) s_wtot[w] = inc;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t ww = 0; ww < w; ++ww) before += s_wtot[ww];
+        s_lbase[t] = before + inc - tot;
     }
     __syncthreads();
+    if constexpr (kStage) {
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        if (rk[r] == 0xffffffffu) continue;
-        uint32_t pos = s_cnt[w][(kk[r] >> shift) & 255u] + rk[r];
-        if (write_keys) keys_out[pos] = kk[r];
-        vals_out[pos] = vv[r];
+        for (int r = 0; r < ITEMS; ++r) {
+            if (rk[r] == 0xffffffffu) continue;
+            const uint32_t d = (kk[r] >> shift) & 255u;
+            const uint32_t lp = s_lbase[d] + s_cnt[w][d] + rk[r];
+            s_k[lp] = kk[r];
+            s_v[lp] = vv[r];
+        }
+        __syncthreads();
+        const uint32_t count = (uint32_t)min((size_t)kTileKeys, (size_t)n - tile_base);
+        for (uint32_t j = threadIdx.x; j < count; j += kBlock) {
+            const uint32_t k = s_k[j];
+            const uint32_t d = (k >> shift) & 255u;
+            const uint32_t pos = s_gbase[d] + (j - s_lbase[d]);
+            if (write_keys) keys_out[pos] = k;
+            vals_out[pos] = s_v[j];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r) {
+            if (rk[r] == 0xffffffffu) continue;
+            const uint32_t d = (kk[r] >> shift) & 255u;
+            const uint32_t pos = s_gbase[d] + s_cnt[w][d] + rk[r];
+            if (write_keys) keys_out[pos] = kk[r];
+            vals_out[pos] = vv[r];
+        }
     }
 }
 
