@@ -171,25 +171,30 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
     for (int u = 0; u < U; ++u) {
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
-        m[u] = c[u] < C ? ld_at(nmask, c[u]) : (uint16_t)0;
     }
-    // Every value load is issued without waiting for the mask (a neighbour
-    // index is clamped into the grid); the mask selects the terms below, so a
-    // wave pays one memory round trip per batch of tiles instead of two.
+    // Every load is issued by every lane, with no branch and without waiting
+    // for the mask: indices are clamped into the grid (a lane past the tile
+    // list reads cell C - 1), and a value is used only for a cell in the grid
+    // whose stencil mask has that term (r only after the first iteration).
+    // A wave thus pays one memory round trip per batch of tiles.
+    uint16_t mv[U];
     float rc[U], dc[U], rn[U][6], dnb[U][6];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const bool ok = c[u] < C;
-        rc[u] = (ok && !first) ? ld_at(r, c[u]) : 0.0f;
-        dc[u] = ok ? ld_at(dprev, c[u]) : 0.0f;
+        const uint32_t cc = c[u] < C ? c[u] : C - 1u;
+        mv[u] = ld_at(nmask, cc);
+        rc[u] = ld_at(r, cc);
+        dc[u] = ld_at(dprev, cc);
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             int32_t nb = (int32_t)c[u] + off[s];
             nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
-            rn[u][s] = (ok && !first) ? ld_at(r, (uint32_t)nb) : 0.0f;
-            dnb[u][s] = ok ? ld_at(dprev, (uint32_t)nb) : 0.0f;
+            rn[u][s] = ld_at(r, (uint32_t)nb);
+            dnb[u][s] = ld_at(dprev, (uint32_t)nb);
         }
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) m[u] = c[u] < C ? mv[u] : (uint16_t)0;
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -229,16 +234,17 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
         const uint32_t t = ti + (uint32_t)u * tw.stride;
         c[u] = t < tw.end ? tile_id(t) * kTile + lane : C;
     }
+    // Unconditional loads at a clamped index, as in phase_a_batch.
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const bool ok = c[u] < C;
+        const uint32_t cc = c[u] < C ? c[u] : C - 1u;
         if (PMODE != kPNone) {
-            pv[u] = ok ? ld_at(p, c[u]) : 0.0f;
-            dv[u] = ok ? ld_at(dn, c[u]) : 0.0f;
+            pv[u] = ld_at(p, cc);
+            dv[u] = ld_at(dn, cc);
         }
-        if (PMODE == kPTwo) dpv[u] = ok ? ld_at(dprev, c[u]) : 0.0f;
-        rv[u] = ok ? ld_at(r, c[u]) : 0.0f;
-        qv[u] = ok ? ld_at(q, c[u]) : 0.0f;
+        if (PMODE == kPTwo) dpv[u] = ld_at(dprev, cc);
+        rv[u] = ld_at(r, cc);
+        qv[u] = ld_at(q, cc);
     }
     double acc = 0.0;
 #pragma unroll
@@ -283,6 +289,17 @@ __device__ __forceinline__ void store_through(float* p, float v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Element i read with an agent-scope relaxed atomic load (an sc1 load): it
+// bypasses the CU's L1 and returns the value at the agent's coherence point,
+// so it sees every store_through whose acknowledgement its writer waited for
+// before the grid barrier it crossed (coherent_sum), with no cache
+// invalidation between.
+template <typename T>
+__device__ __forceinline__ T ld_coherent_at(const T* base, uint32_t i) {
+    return __hip_atomic_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T)),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int U>
 struct ResidentCells {
     uint32_t c[U];
@@ -314,10 +331,12 @@ __device__ __forceinline__ double resident_load(const StepParams& P, const uint1
     return acc;
 }
 
-template <int U>
+// COH reads the neighbours with ld_coherent_at (the fence-free barrier).
+template <int U, bool COH>
 __device__ __forceinline__ double resident_phase_a(const StepParams& P, const float* __restrict__ r,
                                                    const float* __restrict__ dprev, float* __restrict__ dnext,
                                                    float beta, bool first, ResidentCells<U>& rc) {
+    auto ld = [](const float* base, uint32_t i) -> float { return COH ? ld_coherent_at(base, i) : ld_at(base, i); };
     const uint32_t C = P.C;
     const int32_t off[6] = {-1, 1, -(int32_t)P.nx, (int32_t)P.nx, -(int32_t)(P.nx * P.ny), (int32_t)(P.nx * P.ny)};
     float rn[U][6], dnb[U][6];
@@ -328,13 +347,16 @@ __device__ __forceinline__ double resident_phase_a(const StepParams& P, const fl
         // being hoisted out of the solve loop and held in registers throughout.
         uint32_t cu = rc.c[u];
         asm volatile("" : "+v"(cu));
-        const bool ok = cu < C;
+        // Every lane loads (the index is clamped into the grid), so no load
+        // sits behind a branch; a value is used only for a cell in the grid
+        // whose stencil mask has that neighbour, and r only after the first
+        // iteration.
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             int32_t nb = (int32_t)cu + off[s];
             nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
-            rn[u][s] = (ok && !first) ? ld_at(r, (uint32_t)nb) : 0.0f;
-            dnb[u][s] = ok ? ld_at(dprev, (uint32_t)nb) : 0.0f;
+            rn[u][s] = ld(r, (uint32_t)nb);
+            dnb[u][s] = ld(dprev, (uint32_t)nb);
         }
     }
     double acc = 0.0;

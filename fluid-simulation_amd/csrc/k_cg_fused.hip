@@ -12,8 +12,9 @@
 //        batch (solve_blocks).  A workgroup keeps its cells in registers when
 //        they fit (ResidentCells) and streams them otherwise.  Two grid-wide
 //        reductions per iteration: a counter-free exchange after phase A
-//        (grid_exchange) and a memory-ordering one after phase B (grid_sum
-//        or grid_sum_tagged, by workgroup count);
+//        (grid_exchange) and a memory-ordering one after phase B
+//        (grid_sum_tagged, grid_sum or the fence-free coherent_sum, by
+//        workgroup count and residency);
 //      - k_cg_single: the same CG in the single-reduction arrangement (one
 //        sweep and one barrier per iteration), kept as a measured
 //        alternative: its rounding path misses the per-step parity bar;
@@ -211,9 +212,10 @@ __device__ __forceinline__ bool grid_sum(double v, const CgWork& w, uint32_t G, 
 // finished reading epoch e.  The slots start cleared and epochs start at 1.
 // Returns false on abort.
 template <int THREADS>
-__device__ bool grid_exchange_all(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
+__device__ __forceinline__ bool grid_exchange_all(double v, const CgWork& w, uint64_t* slots, uint32_t G, uint32_t epoch,
+                                  double& out) {
     __shared__ int s_abort;
-    uint64_t* half = w.slots + (size_t)(epoch & 1u) * G * 2u;
+    uint64_t* half = slots + (size_t)(epoch & 1u) * G * 2u;
     const double in[1] = {v};
     double bs[1];
     block_sum_nv<THREADS, 1>(in, bs);
@@ -290,7 +292,7 @@ __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* 
 }
 
 template <int THREADS>
-__device__ bool grid_sum_tagged(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
+__device__ __forceinline__ bool grid_sum_tagged(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
     return tagged_sum<THREADS, true>(v, w, w.slots_data, G, epoch, out);
 }
 
@@ -300,7 +302,27 @@ __device__ bool grid_sum_tagged(double v, const CgWork& w, uint32_t G, uint32_t 
 template <int THREADS>
 __device__ __forceinline__ bool grid_exchange(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
     return G <= 64u ? tagged_sum<THREADS, false>(v, w, w.slots, G, epoch, out)
-                    : grid_exchange_all<THREADS>(v, w, G, epoch, out);
+                    : grid_exchange_all<THREADS>(v, w, w.slots, G, epoch, out);
+}
+
+// Fence-free grid barrier + sum for the register-resident solve, in which
+// the only data that crosses workgroups are r and d: written with
+// store_through (agent-scope sc1 stores, written through the L2) and read
+// with ld_coherent_at (agent-scope sc1 loads, which bypass the L1 and read
+// at the coherence point).  Each wave first waits until its own stores are
+// acknowledged (s_waitcnt vmcnt(0): an acknowledged sc1 store is visible at
+// agent scope); the workgroup barrier inside the sum then orders those
+// waits before thread 0 publishes the tagged partial.  A reader issues its
+// loads only after its workgroup has seen every block's tag, so each load
+// returns the value stored before that tag.  These are the rules by which
+// agent-scope atomics are ordered on this target; here they replace the L2
+// write-back and invalidate of a fenced barrier.  It uses the tagged slots
+// and epochs of grid_sum_tagged.
+template <int THREADS>
+__device__ __forceinline__ bool coherent_sum(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    return G <= 64u ? tagged_sum<THREADS, false>(v, w, w.slots_data, G, epoch, out)
+                    : grid_exchange_all<THREADS>(v, w, w.slots_data, G, epoch, out);
 }
 
 // Last-block-done reduction: no block waits; the last to arrive gets the
@@ -337,22 +359,27 @@ __device__ __forceinline__ uint32_t solve_blocks(uint32_t nt, uint32_t per_block
 }
 
 // Form of the reductions that order memory: kDbCounter (grid_sum),
-// kDbTagged (grid_sum_tagged) or kDbAuto: tagged slots up to kTaggedMaxBlocks
-// participating workgroups, counters above.  Measured per iteration on
-// MI355X: 8 workgroups 7.2 vs 9.5 us, 16 about 9.3 vs 11.0, 208 equal, 256
-// (streaming 256^3) 51.0 vs 50.4.
-constexpr int kDbCounter = 0, kDbTagged = 1, kDbAuto = 2;
+// kDbTagged (grid_sum_tagged), kDbCoherent (coherent_sum in the loop when
+// every participating block keeps its cells in registers, else as kDbAuto)
+// or kDbAuto: tagged slots up to kTaggedMaxBlocks participating workgroups;
+// above, coherent_sum when every block is resident and counters otherwise.
+// Measured per iteration on MI355X, tagged vs counter: 8 workgroups 7.2 vs
+// 9.5 us, 16 about 9.3 vs 11.0, 208 equal, 256 (streaming 256^3) 51.0 vs
+// 50.4; coherent vs fenced: 8 workgroups 6.97 vs 6.13 (tagged), 16 8.44 vs
+// 8.05 (tagged), 208 11.4 vs 14.2 (counter).
+constexpr int kDbCounter = 0, kDbTagged = 1, kDbAuto = 2, kDbCoherent = 3;
 constexpr uint32_t kTaggedMaxBlocks = 128;
 template <int DB>
 __device__ __forceinline__ bool use_tagged(uint32_t G) {
-    return DB == kDbTagged || (DB == kDbAuto && G <= kTaggedMaxBlocks);
+    return DB == kDbTagged || ((DB == kDbAuto || DB == kDbCoherent) && G <= kTaggedMaxBlocks);
 }
 
 // The iteration loop of the two-phase solve over one block's tiles; RES
 // keeps the cells in registers (every wave has at most one batch of kU
-// tiles).  Returns false on abort.  Both forms run the same reductions in
-// the same order and the same f32 expressions.
-template <int THREADS, bool EXCH, int DB, bool RES, typename TileId>
+// tiles); COH (with RES in every block) uses coherent_sum and coherent
+// neighbour loads.  Returns false on abort.  All forms run the same
+// reductions in the same order and the same f32 expressions.
+template <int THREADS, bool EXCH, int DB, bool RES, bool COH, typename TileId>
 __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* __restrict__ b, float* __restrict__ p,
                                              float* __restrict__ r, float* __restrict__ d0, float* __restrict__ d1,
                                              float* __restrict__ q, const CgWork& w, uint32_t G, const TileWalk& tw,
@@ -362,6 +389,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
     // The reductions that order memory: counters (grid_sum) or tagged slots
     // (grid_sum_tagged, epochs 1 = r.r of r_0, k + 2 = iteration k).
     auto data_sum = [&](double v, uint32_t epoch, double& out) -> bool {
+        if (COH) return coherent_sum<THREADS>(v, w, G, epoch, out);
         return use_tagged<DB>(G) ? grid_sum_tagged<THREADS>(v, w, G, epoch, out) : grid_sum<THREADS>(v, w, G, out);
     };
     ResidentCells<kU> rc;
@@ -390,7 +418,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
         float* dn_buf = first ? dprev : dnext;
         acc = 0.0;
         if (RES)
-            acc = resident_phase_a<kU>(P, r, dprev, dnext, beta, first, rc);
+            acc = resident_phase_a<kU, COH>(P, r, dprev, dnext, beta, first, rc);
         else
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
                 acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
@@ -462,13 +490,22 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     // A block whose slice gives every wave at most one batch of kU tiles keeps
     // its cells in registers; the others stream the vectors.
     const bool resident = s1 - s0 <= (uint32_t)(THREADS / 64 * kU);
+    // Every block is resident when the largest slice, ceil(nt / G) tiles,
+    // is; the same for all blocks, so they all take the same barriers.
+    const bool all_resident = (nt + G - 1u) / G <= (uint32_t)(THREADS / 64 * kU);
+    const bool coherent = all_resident && (DB == kDbCoherent || (DB == kDbAuto && G > kTaggedMaxBlocks));
     double rs_init = 0.0, rs_old = 0.0;
     uint32_t iters = 0;
-    const bool ok = resident
-                        ? cg_two_phase<THREADS, EXCH, DB, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
-                                                                    rs_old, iters)
-                        : cg_two_phase<THREADS, EXCH, DB, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
-                                                                     rs_old, iters);
+    bool ok;
+    if (coherent)
+        ok = cg_two_phase<THREADS, EXCH, DB, true, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
+                                                         iters);
+    else if (resident)
+        ok = cg_two_phase<THREADS, EXCH, DB, true, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
+                                                          iters);
+    else
+        ok = cg_two_phase<THREADS, EXCH, DB, false, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
+                                                           rs_old, iters);
     if (!ok) {
         if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
         return;
@@ -592,8 +629,9 @@ namespace {
 // Persistent-kernel variant: FLUIDSTEP_CG_FORM=twophase (default) or
 // single; workgroup size 1024 (default) or 512 (FLUIDSTEP_CG_THREADS);
 // for the two-phase form FLUIDSTEP_CG_EXCHANGE=0 keeps grid_sum after
-// phase A, and FLUIDSTEP_CG_DATA_BARRIER=counter|tagged fixes the form of
-// the memory-ordering reductions (default: by workgroup count, use_tagged).
+// phase A, and FLUIDSTEP_CG_DATA_BARRIER=counter|tagged|coherent fixes the
+// form of the memory-ordering reductions (default kDbAuto: by workgroup
+// count and residency).
 struct PersistentVariant {
     int threads;
     bool single;
@@ -606,27 +644,35 @@ template <int T, bool X>
 const void* two_phase_fn_x(int db) {
     if (db == kDbCounter) return (const void*)k_cg_persistent<T, X, kDbCounter>;
     if (db == kDbTagged) return (const void*)k_cg_persistent<T, X, kDbTagged>;
+    if (db == kDbCoherent) return (const void*)k_cg_persistent<T, X, kDbCoherent>;
     return (const void*)k_cg_persistent<T, X, kDbAuto>;
 }
 template <int T>
 const void* two_phase_fn(bool exch, int db) {
     return exch ? two_phase_fn_x<T, true>(db) : two_phase_fn_x<T, false>(db);
 }
+// Workgroup size, form and exchange are fixed per process (the sync area is
+// sized from them); the data-barrier form is read at every launch, so one
+// process can run the same solve under each form.
 PersistentVariant persistent_variant() {
-    static const PersistentVariant v = [] {
+    static const PersistentVariant base = [] {
         PersistentVariant r{1024, false, true, kDbAuto, nullptr};
         if (const char* e = std::getenv("FLUIDSTEP_CG_THREADS")) r.threads = std::atoi(e) == 512 ? 512 : 1024;
         if (const char* e = std::getenv("FLUIDSTEP_CG_FORM")) r.single = std::strcmp(e, "single") == 0;
         if (const char* e = std::getenv("FLUIDSTEP_CG_EXCHANGE")) r.exch = std::atoi(e) != 0;
-        if (const char* e = std::getenv("FLUIDSTEP_CG_DATA_BARRIER"))
-            r.db = std::strcmp(e, "tagged") == 0 ? kDbTagged : (std::strcmp(e, "counter") == 0 ? kDbCounter : kDbAuto);
-        if (r.single)
-            r.fn = r.threads == 1024 ? (const void*)k_cg_single<1024, 2> : (const void*)k_cg_single<512, 2>;
-        else
-            r.fn = r.threads == 1024 ? two_phase_fn<1024>(r.exch, r.db) : two_phase_fn<512>(r.exch, r.db);
         return r;
     }();
-    return v;
+    PersistentVariant r = base;
+    if (const char* e = std::getenv("FLUIDSTEP_CG_DATA_BARRIER"))
+        r.db = std::strcmp(e, "tagged") == 0     ? kDbTagged
+               : std::strcmp(e, "counter") == 0  ? kDbCounter
+               : std::strcmp(e, "coherent") == 0 ? kDbCoherent
+                                                 : kDbAuto;
+    if (r.single)
+        r.fn = r.threads == 1024 ? (const void*)k_cg_single<1024, 2> : (const void*)k_cg_single<512, 2>;
+    else
+        r.fn = r.threads == 1024 ? two_phase_fn<1024>(r.exch, r.db) : two_phase_fn<512>(r.exch, r.db);
+    return r;
 }
 }  // namespace
 

@@ -344,3 +344,35 @@ def test_saturating_p2g_velocities(engine):
     for a in range(3):
         assert bits_equal(saved[a], ref.saved[a]), f"saved axis {a}"
     t.close()
+
+
+@pytest.mark.parametrize("scene_name,forms", [
+    ("dam_break_32", ("tagged", "coherent", "counter")),
+    ("control_64", ("tagged", "coherent")),
+    ("high_density_128", ("counter", "auto", "coherent", "tagged")),
+])
+def test_barrier_forms_give_identical_bits(scene_name, forms, monkeypatch):
+    """The persistent solve's memory-ordering barrier after phase B has three
+    forms: counters with release/acquire (grid_sum), tagged slots with fences
+    (grid_sum_tagged), and the fence-free form of the register-resident solve
+    (coherent_sum: write-through stores and coherent neighbour loads, no L2
+    write-back or invalidate).  All run the same arithmetic in the same
+    order, so a stale neighbour value read by any form would show as a bit
+    difference.  The 128^3 tank solves on 208 workgroups, all resident, where
+    the default ("auto") takes the fence-free form."""
+    sc = getattr(scenes, scene_name)()
+    outs = []
+    for form in forms:
+        monkeypatch.setenv("FLUIDSTEP_CG_DATA_BARRIER", form)
+        say("barrier form", scene_name, form)
+        t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED), sc["pos"], sc["vel"])
+        t.step(3)
+        pos, vel = t.particles()
+        faces = t.faces(fsm.FACE_VELOCITY)
+        outs.append(([t.pressure(), *faces, pos, vel], t.stats()["cg_iterations"]))
+        t.close()
+    base, iters = outs[0]
+    for (arrs, it), form in zip(outs[1:], forms[1:]):
+        assert it == iters, form
+        for i, (a, b) in enumerate(zip(base, arrs)):
+            assert bits_equal(a, b), (form, i)
