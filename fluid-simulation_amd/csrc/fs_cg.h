@@ -156,9 +156,31 @@ __device__ __forceinline__ TileWalk tile_walk(uint32_t nt, uint32_t logical_bloc
     return t;
 }
 
+// Store of a value other blocks read after the next grid barrier: an
+// agent-scope relaxed atomic store goes through the XCD's L2 to memory at
+// once, so the barrier's release finds the line clean instead of writing
+// it back then.
+__device__ __forceinline__ void store_through(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Element i read with an agent-scope relaxed atomic load (an sc1 load): it
+// bypasses the CU's L1 and returns the value at the agent's coherence point,
+// so it sees every store_through whose acknowledgement its writer waited for
+// before the grid barrier it crossed (coherent_sum), with no cache
+// invalidation between.
+template <typename T>
+__device__ __forceinline__ T ld_coherent_at(const T* base, uint32_t i) {
+    return __hip_atomic_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T)),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Phase A for tiles ti, ti+stride, ... (U of them): dn = first ? d : r + beta*d_prev,
-// q = A dn (liquid cells; 0 elsewhere); returns this lane's dn.q.
-template <int U, typename TileId>
+// q = A dn (liquid cells; 0 elsewhere); returns this lane's dn.q.  COH (the
+// fence-free barrier) stores d with store_through and reads the neighbours
+// with ld_coherent_at; a lane's own cells are read back by the wave that
+// wrote them.
+template <int U, bool COH = false, typename TileId>
 __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint16_t* __restrict__ nmask,
                                                 const float* __restrict__ r, const float* __restrict__ dprev,
                                                 float* __restrict__ dnext, float* __restrict__ q, float beta,
@@ -189,8 +211,8 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
         for (int s = 0; s < 6; ++s) {
             int32_t nb = (int32_t)c[u] + off[s];
             nb = nb < 0 ? 0 : (nb >= (int32_t)C ? (int32_t)C - 1 : nb);
-            rn[u][s] = ld_at(r, (uint32_t)nb);
-            dnb[u][s] = ld_at(dprev, (uint32_t)nb);
+            rn[u][s] = COH ? ld_coherent_at(r, (uint32_t)nb) : ld_at(r, (uint32_t)nb);
+            dnb[u][s] = COH ? ld_coherent_at(dprev, (uint32_t)nb) : ld_at(dprev, (uint32_t)nb);
         }
     }
 #pragma unroll
@@ -208,7 +230,12 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
                 if ((m[u] >> (3 + s)) & 1u) sum += first ? dnb[u][s] : rn[u][s] + beta * dnb[u][s];
             qc = (float)(m[u] & 7u) * dn - sum;
         }
-        if (!first) st_at(dnext, c[u], dn);
+        if (!first) {
+            if (COH)
+                store_through(&dnext[c[u]], dn);
+            else
+                st_at(dnext, c[u], dn);
+        }
         st_at(q, c[u], qc);
         acc += (double)dn * (double)qc;
     }
@@ -220,8 +247,9 @@ __device__ __forceinline__ double phase_a_batch(const StepParams& P, const uint1
 // + alpha dn, the deferred update of iteration k-1 and the one of iteration
 // k in their own order (the same f32 operations as one update per
 // iteration, so deferring every other update changes no bit).
+// COH stores r with store_through (the fence-free barrier).
 constexpr int kPNone = 0, kPOne = 1, kPTwo = 2;
-template <int U, int PMODE, typename TileId>
+template <int U, int PMODE, bool COH = false, typename TileId>
 __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float* __restrict__ dn,
                                                 const float* __restrict__ q, float* __restrict__ p,
                                                 float* __restrict__ r, float alpha, const float* __restrict__ dprev,
@@ -253,7 +281,10 @@ __device__ __forceinline__ double phase_b_batch(const StepParams& P, const float
         if (PMODE == kPOne) st_at(p, c[u], pv[u] + alpha * dv[u]);
         if (PMODE == kPTwo) st_at(p, c[u], (pv[u] + alpha_prev * dpv[u]) + alpha * dv[u]);
         const float rn = rv[u] - alpha * qv[u];
-        st_at(r, c[u], rn);
+        if (COH)
+            store_through(&r[c[u]], rn);
+        else
+            st_at(r, c[u], rn);
         acc += (double)rn * (double)rn;
     }
     return acc;
@@ -281,25 +312,6 @@ __device__ __forceinline__ void p_update_batch(const StepParams& P, const float*
 // nothing and stores r; p is stored once after the loop.  The f32
 // expressions are those of phase_a_batch / phase_b_batch, so both forms give
 // the same bits.
-// Store of a value other blocks read after the next grid barrier: an
-// agent-scope relaxed atomic store goes through the XCD's L2 to memory at
-// once, so the barrier's release finds the line clean instead of writing
-// it back then.
-__device__ __forceinline__ void store_through(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Element i read with an agent-scope relaxed atomic load (an sc1 load): it
-// bypasses the CU's L1 and returns the value at the agent's coherence point,
-// so it sees every store_through whose acknowledgement its writer waited for
-// before the grid barrier it crossed (coherent_sum), with no cache
-// invalidation between.
-template <typename T>
-__device__ __forceinline__ T ld_coherent_at(const T* base, uint32_t i) {
-    return __hip_atomic_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T)),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <int U>
 struct ResidentCells {
     uint32_t c[U];

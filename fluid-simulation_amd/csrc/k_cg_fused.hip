@@ -305,8 +305,8 @@ __device__ __forceinline__ bool grid_exchange(double v, const CgWork& w, uint32_
                     : grid_exchange_all<THREADS>(v, w, w.slots, G, epoch, out);
 }
 
-// Fence-free grid barrier + sum for the register-resident solve, in which
-// the only data that crosses workgroups are r and d: written with
+// Fence-free grid barrier + sum for the two-phase solve, in which the only
+// data that crosses workgroups are r and d (neighbour values): written with
 // store_through (agent-scope sc1 stores, written through the L2) and read
 // with ld_coherent_at (agent-scope sc1 loads, which bypass the L1 and read
 // at the coherence point).  Each wave first waits until its own stores are
@@ -359,14 +359,14 @@ __device__ __forceinline__ uint32_t solve_blocks(uint32_t nt, uint32_t per_block
 }
 
 // Form of the reductions that order memory: kDbCounter (grid_sum),
-// kDbTagged (grid_sum_tagged), kDbCoherent (coherent_sum in the loop when
-// every participating block keeps its cells in registers, else as kDbAuto)
-// or kDbAuto: tagged slots up to kTaggedMaxBlocks participating workgroups;
-// above, coherent_sum when every block is resident and counters otherwise.
+// kDbTagged (grid_sum_tagged), kDbCoherent (coherent_sum in the loop) or
+// kDbAuto: tagged slots up to kTaggedMaxBlocks participating workgroups,
+// coherent_sum above.
 // Measured per iteration on MI355X, tagged vs counter: 8 workgroups 7.2 vs
 // 9.5 us, 16 about 9.3 vs 11.0, 208 equal, 256 (streaming 256^3) 51.0 vs
 // 50.4; coherent vs fenced: 8 workgroups 6.97 vs 6.13 (tagged), 16 8.44 vs
-// 8.05 (tagged), 208 11.4 vs 14.2 (counter).
+// 8.05 (tagged), 208 resident 11.4 vs 14.1 (counter), 256 streaming 41.0 vs
+// 45.6 (counter).
 constexpr int kDbCounter = 0, kDbTagged = 1, kDbAuto = 2, kDbCoherent = 3;
 constexpr uint32_t kTaggedMaxBlocks = 128;
 template <int DB>
@@ -376,7 +376,7 @@ __device__ __forceinline__ bool use_tagged(uint32_t G) {
 
 // The iteration loop of the two-phase solve over one block's tiles; RES
 // keeps the cells in registers (every wave has at most one batch of kU
-// tiles); COH (with RES in every block) uses coherent_sum and coherent
+// tiles); COH uses coherent_sum, write-through r and d stores and coherent
 // neighbour loads.  Returns false on abort.  All forms run the same
 // reductions in the same order and the same f32 expressions.
 template <int THREADS, bool EXCH, int DB, bool RES, bool COH, typename TileId>
@@ -421,7 +421,7 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
             acc = resident_phase_a<kU, COH>(P, r, dprev, dnext, beta, first, rc);
         else
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_a_batch<kU>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
+                acc += phase_a_batch<kU, COH>(P, w.nmask, r, dprev, dnext, q, beta, first, tw, ti, tile_id);
         double dq;
         const bool ok = EXCH ? grid_exchange<THREADS>(acc, w, G, it + 1u, dq) : grid_sum<THREADS>(acc, w, G, dq);
         if (!ok) return false;
@@ -431,10 +431,12 @@ __device__ __forceinline__ bool cg_two_phase(const StepParams& P, const float* _
             acc = resident_phase_b<kU>(P, r, alpha, rc);
         else if (it & 1u)
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_b_batch<kU, kPTwo>(P, dn_buf, q, p, r, alpha, dprev, alpha_prev, tw, ti, tile_id);
+                acc += phase_b_batch<kU, kPTwo, COH>(P, dn_buf, q, p, r, alpha, dprev, alpha_prev, tw, ti,
+                                                             tile_id);
         else
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_b_batch<kU, kPNone>(P, dn_buf, q, p, r, alpha, nullptr, 0.0f, tw, ti, tile_id);
+                acc += phase_b_batch<kU, kPNone, COH>(P, dn_buf, q, p, r, alpha, nullptr, 0.0f, tw, ti,
+                                                              tile_id);
         alpha_prev = alpha;
         last_alpha = alpha;
         last_dn = dn_buf;
@@ -490,16 +492,18 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     // A block whose slice gives every wave at most one batch of kU tiles keeps
     // its cells in registers; the others stream the vectors.
     const bool resident = s1 - s0 <= (uint32_t)(THREADS / 64 * kU);
-    // Every block is resident when the largest slice, ceil(nt / G) tiles,
-    // is; the same for all blocks, so they all take the same barriers.
-    const bool all_resident = (nt + G - 1u) / G <= (uint32_t)(THREADS / 64 * kU);
-    const bool coherent = all_resident && (DB == kDbCoherent || (DB == kDbAuto && G > kTaggedMaxBlocks));
+    // The barrier form depends on DB and G only, so every block takes the
+    // same one; resident and streaming blocks share r and d the same way.
+    const bool coherent = DB == kDbCoherent || (DB == kDbAuto && G > kTaggedMaxBlocks);
     double rs_init = 0.0, rs_old = 0.0;
     uint32_t iters = 0;
     bool ok;
-    if (coherent)
+    if (coherent && resident)
         ok = cg_two_phase<THREADS, EXCH, DB, true, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
                                                          iters);
+    else if (coherent)
+        ok = cg_two_phase<THREADS, EXCH, DB, false, true>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
+                                                          iters);
     else if (resident)
         ok = cg_two_phase<THREADS, EXCH, DB, true, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init, rs_old,
                                                           iters);

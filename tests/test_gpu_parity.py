@@ -346,12 +346,13 @@ def test_saturating_p2g_velocities(engine):
     t.close()
 
 
-@pytest.mark.parametrize("scene_name,forms", [
-    ("dam_break_32", ("tagged", "coherent", "counter")),
-    ("control_64", ("tagged", "coherent")),
-    ("high_density_128", ("counter", "auto", "coherent", "tagged")),
+@pytest.mark.parametrize("scene_name,forms,blocks", [
+    ("dam_break_32", ("tagged", "coherent", "counter"), None),
+    ("control_64", ("tagged", "coherent"), None),
+    ("control_64", ("tagged", "coherent", "counter"), 8),
+    ("high_density_128", ("counter", "auto", "coherent", "tagged"), None),
 ])
-def test_barrier_forms_give_identical_bits(scene_name, forms, monkeypatch):
+def test_barrier_forms_give_identical_bits(scene_name, forms, blocks, monkeypatch):
     """The persistent solve's memory-ordering barrier after phase B has three
     forms: counters with release/acquire (grid_sum), tagged slots with fences
     (grid_sum_tagged), and the fence-free form of the register-resident solve
@@ -359,7 +360,11 @@ def test_barrier_forms_give_identical_bits(scene_name, forms, monkeypatch):
     write-back or invalidate).  All run the same arithmetic in the same
     order, so a stale neighbour value read by any form would show as a bit
     difference.  The 128^3 tank solves on 208 workgroups, all resident, where
-    the default ("auto") takes the fence-free form."""
+    the default ("auto") takes the fence-free form; the 64^3 tank capped at 8
+    workgroups streams its vectors (the fence-free form then stores r and d
+    through and reads neighbours coherently in the streaming phases)."""
+    if blocks is not None:
+        monkeypatch.setenv("FLUIDSTEP_CG_BLOCKS", str(blocks))
     sc = getattr(scenes, scene_name)()
     outs = []
     for form in forms:
