@@ -415,7 +415,7 @@ FS_API int fs_batch_create(const fs_params* params, const float* const* pos_xyz,
         }
         FS_HIP(bt->alloc(bt->cg.partials, 2 * ((size_t)bt->cg_max_blocks + 8)));
         FS_HIP(bt->alloc(bt->cg.flags, 2 * ((size_t)bt->cg_max_blocks + 8)));
-        bt->cg.slot_words = (uint32_t)(8 * ((size_t)bt->cg_max_blocks + 8));
+        bt->cg.slot_words = (uint32_t)(16 * ((size_t)bt->cg_max_blocks + 8));
         FS_HIP(bt->alloc(bt->cg.slots, bt->cg.slot_words / 2));
         FS_HIP(bt->alloc(bt->d_tanks, ntanks));
         if (const char* env = std::getenv("FLUIDSTEP_BATCH_GRAPH")) bt->use_graph = std::atoi(env) != 0;

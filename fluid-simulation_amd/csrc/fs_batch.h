@@ -48,7 +48,8 @@ struct BatchCgWork {
     uint32_t* arrive;      // [9 * 32]: eight group counters and the grid counter, one 128-byte line each
     uint32_t* gen;         // [32]: grid barrier generation
     uint32_t* abort_flag;  // [1]
-    uint64_t* slots;       // [2 * grid * 2] epoch-tagged partials of the exchanges, cleared per solve
+    uint64_t* slots;       // [2 x (2 * grid * 2)] epoch-tagged partials: the exchange after phase A, then
+                           // the fence-free barrier after phase B; cleared per solve
     uint32_t slot_words;   // u32 words of `slots`
     BatchCgState* state;   // [T]
 };

@@ -11,6 +11,8 @@
 // a group runs that tank's mixed-precision CG over the tank's liquid tiles
 // and reduces over its own partial slots, while the grid barriers are shared
 // by all tanks (lock step); a tank that has converged idles at the barriers.
+#include <cstdlib>
+#include <cstring>
 #include "fs_batch.h"
 #include "fs_cell.h"
 #include "fs_gather.h"
@@ -345,16 +347,17 @@ __device__ bool lockstep_sum(double v, uint32_t flag, const BatchCgWork& w, uint
 
 // The reduction after phase A without counters or fences (phase B then reads
 // only data its own lanes wrote): every workgroup publishes its partial and
-// flag in its epoch-tagged slot (fs_cg.h) and reads all G slots; the tank
-// total and the OR of the flags are formed as in lockstep_sum.  Slots
-// alternate between two halves by epoch parity (a workgroup publishes e+2 only
-// after all have published e+1, i.e. finished reading e).
+// flag in its epoch-tagged slot (fs_cg.h) in `slots` (2 x G slots) and reads
+// all G slots; the tank total and the OR of the flags are formed as in
+// lockstep_sum.  Slots alternate between two halves by epoch parity (a
+// workgroup publishes e+2 only after all have published e+1, i.e. finished
+// reading e).
 template <int THREADS>
-__device__ bool lockstep_exchange(double v, uint32_t flag, const BatchCgWork& w, uint32_t slot, uint32_t base,
-                                  uint32_t nb, uint32_t epoch, double& out, bool& any) {
+__device__ bool lockstep_exchange(double v, uint32_t flag, const BatchCgWork& w, uint64_t* slots, uint32_t slot,
+                                  uint32_t base, uint32_t nb, uint32_t epoch, double& out, bool& any) {
     __shared__ int s_abort;
     const uint32_t G = gridDim.x;
-    uint64_t* half = w.slots + (size_t)(epoch & 1u) * G * 2u;
+    uint64_t* half = slots + (size_t)(epoch & 1u) * G * 2u;
     const double in[1] = {v};
     double bs[1];
     block_sum_nv<THREADS, 1>(in, bs);
@@ -382,10 +385,28 @@ __device__ bool lockstep_exchange(double v, uint32_t flag, const BatchCgWork& w,
     return true;
 }
 
+// The fence-free grid barrier after phase B (coherent_sum of the single-tank
+// solver): the only data that cross workgroups in the loop are r and d,
+// written with store_through and read at neighbours with ld_coherent_at
+// (phase_a_batch / phase_b_batch with COH).  Each wave waits until its
+// stores are acknowledged (s_waitcnt vmcnt(0); an acknowledged sc1 store is
+// visible at agent scope) before the workgroup barrier inside the exchange,
+// after which thread 0 publishes; a workgroup reads neighbours only after it
+// has seen every slot's tag.  Its slots follow the exchange's in `slots`,
+// with their own epochs.
+template <int THREADS>
+__device__ __forceinline__ bool lockstep_coherent(double v, uint32_t flag, const BatchCgWork& w, uint32_t slot,
+                                                  uint32_t base, uint32_t nb, uint32_t epoch, double& out,
+                                                  bool& any) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    return lockstep_exchange<THREADS>(v, flag, w, w.slots + 4u * gridDim.x, slot, base, nb, epoch, out, any);
+}
+
 // Workgroups of a tank take its slice of the liquid-tile list; workgroups
 // whose tank has converged (or padding workgroups, which have none) keep
-// meeting the barriers with zero partials until no tank iterates.
-template <int U>
+// meeting the barriers with zero partials until no tank iterates.  COH takes
+// lockstep_coherent after phase B instead of lockstep_sum.
+template <int U, bool COH>
 __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict__ T, const WorkItem* __restrict__ wl,
                                                       const float* __restrict__ b, float* __restrict__ p,
                                                       float* __restrict__ r, float* __restrict__ d0,
@@ -462,10 +483,10 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
         if (active) {
             ++iters;
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_a_batch<U>(P, nm, rr_, dprev, dnext, q_, beta, first, tw, ti, tile_id);
+                acc += phase_a_batch<U, COH>(P, nm, rr_, dprev, dnext, q_, beta, first, tw, ti, tile_id);
         }
         double dq;
-        if (!lockstep_exchange<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, it + 1u, dq, any)) {
+        if (!lockstep_exchange<kBCgThreads>(acc, active ? 1u : 0u, w, w.slots, slot, base, nb, it + 1u, dq, any)) {
             report(rs_init, rs_old, 0.0, iters, 1u);
             return;
         }
@@ -474,12 +495,15 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
         acc = 0.0;
         if (active && (it & 1u))
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_b_batch<U, kPTwo>(P, dn_buf, q_, pp, rr_, alpha, dprev, alpha_prev, tw, ti, tile_id);
+                acc += phase_b_batch<U, kPTwo, COH>(P, dn_buf, q_, pp, rr_, alpha, dprev, alpha_prev, tw, ti,
+                                                    tile_id);
         else if (active)
             for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
-                acc += phase_b_batch<U, kPNone>(P, dn_buf, q_, pp, rr_, alpha, nullptr, 0.0f, tw, ti, tile_id);
+                acc += phase_b_batch<U, kPNone, COH>(P, dn_buf, q_, pp, rr_, alpha, nullptr, 0.0f, tw, ti, tile_id);
         double rr;
-        if (!lockstep_sum<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, rr, any)) {
+        const bool okb = COH ? lockstep_coherent<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, it + 1u, rr, any)
+                             : lockstep_sum<kBCgThreads>(acc, active ? 1u : 0u, w, slot, base, nb, rr, any);
+        if (!okb) {
             report(rs_init, rs_old, 0.0, iters, 1u);
             return;
         }
@@ -523,7 +547,8 @@ int batch_cg_max_blocks() {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     int lo = 1 << 30;
     // The smallest occupancy over the unroll variants, so every variant fits.
-    const void* fns[3] = {(const void*)kb_cg<4>, (const void*)kb_cg<2>, (const void*)kb_cg<1>};
+    const void* fns[6] = {(const void*)kb_cg<4, true>,  (const void*)kb_cg<2, true>,  (const void*)kb_cg<1, true>,
+                          (const void*)kb_cg<4, false>, (const void*)kb_cg<2, false>, (const void*)kb_cg<1, false>};
     for (const void* f : fns) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBCgThreads, 0) != hipSuccess) return 0;
@@ -537,7 +562,16 @@ hipError_t launch_batch_cg(const BatchTank* T, const WorkItem* wl, uint32_t nwl,
     if (nwl == 0) return hipSuccess;
     void* args[] = {(void*)&T, (void*)&wl, (void*)&b, (void*)&p, (void*)&r,
                     (void*)&d0, (void*)&d1, (void*)&q, (void*)&w};
-    const void* f = cg_unroll == 1 ? (const void*)kb_cg<2> : (cg_unroll == 2 ? (const void*)kb_cg<1> : (const void*)kb_cg<4>);
+    // FLUIDSTEP_BATCH_DATA_BARRIER=counter takes lockstep_sum after phase B
+    // (read at every launch); the default is lockstep_coherent.
+    const char* e = std::getenv("FLUIDSTEP_BATCH_DATA_BARRIER");
+    const bool coh = !(e && std::strcmp(e, "counter") == 0);
+    const void* f = coh ? (cg_unroll == 1   ? (const void*)kb_cg<2, true>
+                           : cg_unroll == 2 ? (const void*)kb_cg<1, true>
+                                            : (const void*)kb_cg<4, true>)
+                        : (cg_unroll == 1   ? (const void*)kb_cg<2, false>
+                           : cg_unroll == 2 ? (const void*)kb_cg<1, false>
+                                            : (const void*)kb_cg<4, false>);
     return hipLaunchCooperativeKernel(f, dim3(nwl), dim3(kBCgThreads), args, 0, s);
 }
 

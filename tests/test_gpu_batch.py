@@ -201,3 +201,32 @@ def test_batch_of_the_benchmark_tanks_one_substep():
         perm = b.permutation(t)
         assert np.array_equal(np.sort(perm), np.arange(len(sc["pos"]), dtype=np.uint32))
     b.close()
+
+
+@pytest.mark.parametrize("which,unroll", [("small", 0), ("small", 1), ("small", 2), ("benchmark", 0)])
+def test_batch_barrier_forms_give_identical_bits(which, unroll, monkeypatch):
+    """The batched solve's barrier after phase B: counters with
+    release/acquire (lockstep_sum) or the fence-free form (lockstep_coherent:
+    write-through r and d, coherent neighbour loads).  Both run the same
+    arithmetic in the same order, so a stale neighbour value would show as a
+    bit difference in some tank."""
+    tanks = small_batch() if which == "small" else scenes.batched_tanks()
+    outs = []
+    for form in ("counter", "coherent"):
+        monkeypatch.setenv("FLUIDSTEP_BATCH_DATA_BARRIER", form)
+        say("batch barrier form", which, unroll, form)
+        b = make_batch(tanks)
+        if unroll:
+            b.set_tuning(cg_unroll=unroll)
+        b.step(2 if which == "small" else 1)
+        b.synchronize()
+        per = []
+        for t in range(len(tanks)):
+            gp, gv = b.particles(t)
+            per.append(([b.pressure(t), *b.faces(t, fsm.FACE_VELOCITY), gp, gv], b.stats(t)["cg_iterations"]))
+        outs.append(per)
+        b.close()
+    for t, ((a_arrs, a_it), (b_arrs, b_it)) in enumerate(zip(outs[0], outs[1])):
+        assert a_it == b_it, t
+        for i, (x, y) in enumerate(zip(a_arrs, b_arrs)):
+            assert bits_equal(x, y), (t, i)
