@@ -56,7 +56,7 @@ __device__ __forceinline__ double sum_partials(const double* partials, uint32_t 
     return s_tree[0];
 }
 
-// Block partials of NV values: wave sums (lane 0's butterfly result), then
+// Block partials of NV values: wave sums (wave_sum, one fixed order), then
 // thread 0 adds the waves in order.  Valid in thread 0 only.
 template <int THREADS, int NV>
 __device__ __forceinline__ void block_sum_nv(const double (&v)[NV], double (&bs)[NV]) {
@@ -79,8 +79,8 @@ __device__ __forceinline__ void block_sum_nv(const double (&v)[NV], double (&bs)
 }
 
 // Total of G published partials in one fixed order: thread i has added
-// partials i, i + THREADS, ... into `loaded`; waves reduce by butterfly
-// (lane 0's result); thread 0 adds the wave results in order.  Every block
+// partials i, i + THREADS, ... into `loaded`; waves reduce with wave_sum;
+// thread 0 adds the wave results in order.  Every block
 // therefore forms the bit-identical total, returned to all threads.
 template <int THREADS, int NV>
 __device__ __forceinline__ void reduce_loaded(const double (&loaded)[NV], double (&out)[NV]) {

@@ -72,10 +72,36 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : ((~0ull) >> (64 - l));
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// v of the partner lane given by DPP control CTRL (a permutation within rows
+// of 16 lanes), both 32-bit halves.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Sum of v over the wave's 64 lanes, returned to every lane; every lane must
+// be active.  Four DPP steps add the partner lane^1, lane^2, the mirror
+// within 8 lanes and the mirror within 16: each partner adds the same two
+// values in the other order, so after every step all lanes of a row of 16
+// hold the identical bits, and after the fourth, the row's sum.  The row
+// sums (lanes 0, 16, 32, 48) are then added as (r0 + r1) + (r2 + r3).  The
+// order is fixed, so equal inputs give equal totals in every wave.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // bf16 storage of particle velocity: round-to-nearest-even on store.

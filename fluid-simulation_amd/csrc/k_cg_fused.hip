@@ -243,9 +243,9 @@ __device__ __forceinline__ bool grid_exchange_all(double v, const CgWork& w, uin
 
 // Sum over tagged slots read by wave 0 alone.  Thread 0 publishes the
 // block's partial in its epoch-tagged slot; wave 0 reads all G slots (lane l
-// takes slots l, l+64, ...) until each carries the epoch.  The total is lane
-// 0's butterfly over the lanes' in-order sums: one fixed order, so every
-// block holds the identical total.  Slots alternate by epoch parity as in
+// takes slots l, l+64, ...) until each carries the epoch.  The total is the
+// wave_sum of the lanes' in-order sums: one fixed order, so every block
+// holds the identical total.  Slots alternate by epoch parity as in
 // grid_exchange_all.
 // FENCED makes it a grid barrier as well (grid_sum_tagged): every block's
 // stores before it are visible to every block after it.  Thread 0 issues a
