@@ -241,12 +241,13 @@ __device__ __forceinline__ bool grid_exchange_all(double v, const CgWork& w, uin
     return true;
 }
 
-// Sum over tagged slots read by wave 0 alone.  Thread 0 publishes the
-// block's partial in its epoch-tagged slot; wave 0 reads all G slots (lane l
-// takes slots l, l+64, ...) until each carries the epoch.  The total is the
-// wave_sum of the lanes' in-order sums: one fixed order, so every block
-// holds the identical total.  Slots alternate by epoch parity as in
-// grid_exchange_all.
+// Sum over tagged slots read by wave 0 alone, for G <= kPollSlots * 64.
+// Thread 0 publishes the block's partial in its epoch-tagged slot; lane l
+// of wave 0 polls slots l, l + 64, ... all at once (their tags are loaded
+// together each round) until each carries the epoch.  The total is formed
+// in reduce_loaded's order, as if thread i held slot i: the wave sums of
+// slots [64k, 64k + 64), added in k order from 0.0.  So every block, and
+// every reduction of this file, holds the identical total.  Slots alternate by epoch parity as in grid_exchange_all.
 // FENCED makes it a grid barrier as well (grid_sum_tagged): every block's
 // stores before it are visible to every block after it.  Thread 0 issues a
 // release fence (writes the XCD's L2 back) before publishing; the polling
@@ -254,6 +255,7 @@ __device__ __forceinline__ bool grid_exchange_all(double v, const CgWork& w, uin
 // its own reads of the tags it waited for), and the workgroup barrier orders
 // the other waves after it.  The fenced form has its own slot area and epoch
 // sequence.
+constexpr uint32_t kPollSlots = 4;
 template <int THREADS, bool FENCED>
 __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* slots, uint32_t G, uint32_t epoch,
                                            double& out) {
@@ -269,21 +271,55 @@ __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* 
         publish_tagged(half, blockIdx.x, epoch, bs[0], 0u);
     }
     if (threadIdx.x < 64) {
-        double acc = 0.0;
+        const uint32_t lane = threadIdx.x;
+        double x[kPollSlots];
+        uint32_t pending = 0;  // bit k: slot lane + 64 k not yet seen at this epoch
+#pragma unroll
+        for (uint32_t k = 0; k < kPollSlots; ++k) {
+            x[k] = 0.0;
+            if (lane + 64u * k < G) pending |= 1u << k;
+        }
+        const long long t0 = wall_clock64();
         bool ok = true;
-        for (uint32_t i = threadIdx.x; i < G; i += 64) {
-            double x;
-            uint32_t f;
-            if (!poll_tagged(half, i, epoch, w.abort_flag, x, f)) {
+        for (uint32_t spin = 0; pending; ++spin) {
+            // Rows k of slots that exist (64 k < G, the same for all lanes)
+            // are loaded by every lane, at an index clamped to the last slot
+            // (in bounds); only pending slots take the value.
+            uint64_t hi[kPollSlots], lo[kPollSlots];
+#pragma unroll
+            for (uint32_t k = 0; k < kPollSlots; ++k) {
+                hi[k] = lo[k] = 0;
+                if (64u * k < G) {
+                    const uint32_t i = lane + 64u * k < G ? lane + 64u * k : G - 1u;
+                    hi[k] = __hip_atomic_load(half + 2u * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lo[k] = __hip_atomic_load(half + 2u * i + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kPollSlots; ++k) {
+                if ((pending >> k) & 1u && (uint32_t)(hi[k] >> 32) == epoch && (uint32_t)(lo[k] >> 33) == epoch) {
+                    x[k] = 0.0 + __longlong_as_double((long long)(((hi[k] & 0xffffffffull) << 32) |
+                                                                  (lo[k] & 0xffffffffull)));
+                    pending &= ~(1u << k);
+                }
+            }
+            if (pending && (spin & 31u) == 31u &&
+                (ld_relaxed(w.abort_flag) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+                __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = false;
                 break;
             }
-            acc += x;
         }
         if (!ok) s_abort = 1;
         if (FENCED) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const double t = wave_sum(acc);
-        if (threadIdx.x == 0) s_out = t;
+        // Rows past G hold +0.0 in every lane and would add +0.0; a sum that
+        // starts at +0.0 is never -0.0, so leaving those terms out changes
+        // no bit of reduce_loaded's total.
+        double t = 0.0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPollSlots; ++k)
+            if (64u * k < G) t += wave_sum(x[k]);
+        if (lane == 0) s_out = t;
     }
     __syncthreads();
     if (s_abort) return false;
@@ -293,16 +329,18 @@ __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* 
 
 template <int THREADS>
 __device__ __forceinline__ bool grid_sum_tagged(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
-    return tagged_sum<THREADS, true>(v, w, w.slots_data, G, epoch, out);
+    // tagged_sum reads at most kPollSlots * 64 slots; larger grids (512-thread
+    // workgroups) take the counter barrier.
+    return G <= kPollSlots * 64u ? tagged_sum<THREADS, true>(v, w, w.slots_data, G, epoch, out)
+                                 : grid_sum<THREADS>(v, w, G, out);
 }
 
-// The phase-A exchange: up to 64 workgroups, wave 0 alone reads the slots
-// (one per lane; fewer workgroup barriers); above, every thread reads one
-// (measured per iteration: 8 workgroups 7.05 vs 7.27 us, 208: 16.1 vs 15.15).
+// The phase-A exchange: up to kPollSlots * 64 workgroups, wave 0 alone
+// reads the slots (fewer workgroup barriers); above, every thread reads one.
 template <int THREADS>
 __device__ __forceinline__ bool grid_exchange(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
-    return G <= 64u ? tagged_sum<THREADS, false>(v, w, w.slots, G, epoch, out)
-                    : grid_exchange_all<THREADS>(v, w, w.slots, G, epoch, out);
+    return G <= kPollSlots * 64u ? tagged_sum<THREADS, false>(v, w, w.slots, G, epoch, out)
+                                 : grid_exchange_all<THREADS>(v, w, w.slots, G, epoch, out);
 }
 
 // Fence-free grid barrier + sum for the two-phase solve, in which the only
@@ -321,8 +359,8 @@ __device__ __forceinline__ bool grid_exchange(double v, const CgWork& w, uint32_
 template <int THREADS>
 __device__ __forceinline__ bool coherent_sum(double v, const CgWork& w, uint32_t G, uint32_t epoch, double& out) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    return G <= 64u ? tagged_sum<THREADS, false>(v, w, w.slots_data, G, epoch, out)
-                    : grid_exchange_all<THREADS>(v, w, w.slots_data, G, epoch, out);
+    return G <= kPollSlots * 64u ? tagged_sum<THREADS, false>(v, w, w.slots_data, G, epoch, out)
+                                 : grid_exchange_all<THREADS>(v, w, w.slots_data, G, epoch, out);
 }
 
 // Last-block-done reduction: no block waits; the last to arrive gets the
