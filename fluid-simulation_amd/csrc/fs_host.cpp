@@ -44,10 +44,14 @@ struct Row {
     int apply;
 };
 const Row kRows[] = {
+    {"scene.preset", true, 0, 0, 2, FS_APPLY_RESET},
+    {"scene.fill_level", false, 20.0, 0.0, 100.0, FS_APPLY_RESET},
+    {"scene.drop_height", false, 1.0, 0.0, 1.0, FS_APPLY_RESET},
     {"grid.res_x", true, 80, 16, 128, FS_APPLY_RESET},
     {"grid.res_y", true, 40, 16, 128, FS_APPLY_RESET},
     {"grid.res_z", true, 80, 16, 128, FS_APPLY_RESET},
     {"particles.density", false, 10.0, 1.0, 32.0, FS_APPLY_RESET},
+    {"particles.count", true, 0, 0, 134217728, FS_APPLY_RESET},
     {"physics.gravity", false, -9.81, -40.0, 40.0, FS_APPLY_LIVE},
     {"physics.fixed_dt", false, 1.0 / 120.0, 1.0 / 480.0, 1.0 / 30.0, FS_APPLY_RESET},
     {"physics.sim_speed", false, 1.0, 0.05, 1.0, FS_APPLY_LIVE},

@@ -43,6 +43,7 @@ EXPORTED = (
     "fs_batch_buffer_bytes", "fs_batch_read_particles", "fs_batch_read_faces",
     "fs_batch_read_cells_f32", "fs_batch_read_cells_u32", "fs_batch_read_permutation",
     "fs_batch_get_stats", "fs_forward_checkpointed", "fs_step_adjoint",
+    "fs_scene_preset_blocks", "fs_scene_from_registry", "fs_scene_seed", "fs_create_from_settings",
 )
 
 
@@ -124,6 +125,29 @@ class Tuning(C.Structure):
         }
 
 
+class LiquidBlock(C.Structure):
+    """fs_liquid_block: LiquidBlock (src/scene/mod.rs:61-75), normalized tank space."""
+    _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3)]
+
+
+SCENE_FALLING_BLOB, SCENE_DAM_BREAK, SCENE_DOUBLE_SPLASH = 0, 1, 2
+SCENE_MAX_BLOCKS = 8
+
+
+class Scene(C.Structure):
+    """fs_scene: SceneConfig (src/scene/mod.rs:84-91) plus its derived counts."""
+    _fields_ = [
+        ("preset", C.c_uint32), ("res", C.c_uint32 * 3), ("nblocks", C.c_uint32),
+        ("blocks", LiquidBlock * SCENE_MAX_BLOCKS), ("particle_count", C.c_uint32),
+        ("effective_density", C.c_float), ("generated_count", C.c_uint32),
+    ]
+
+    def blocks_array(self) -> np.ndarray:
+        """(nblocks, 6) float32: min xyz, max xyz."""
+        return np.array([list(self.blocks[i].min) + list(self.blocks[i].max) for i in range(self.nblocks)],
+                        np.float32).reshape(-1, 6)
+
+
 _lib = None
 
 
@@ -188,6 +212,12 @@ def lib():
         L.fs_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
         L.fs_forward_checkpointed.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.fs_step_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.fs_scene_preset_blocks.argtypes = [C.c_uint32, C.c_float, C.c_float, C.POINTER(LiquidBlock),
+                                             C.POINTER(C.c_uint32)]
+        L.fs_scene_from_registry.argtypes = [vp, C.POINTER(Scene)]
+        L.fs_scene_seed.argtypes = [C.POINTER(Scene), C.c_float, C.POINTER(C.c_float), vp, C.c_uint32,
+                                    C.POINTER(C.c_uint32)]
+        L.fs_create_from_settings.argtypes = [vp, C.POINTER(Scene), C.POINTER(vp)]
         _lib = L
     return _lib
 
@@ -256,8 +286,50 @@ class Registry:
         return p
 
 
+def scene_preset_blocks(preset: int, drop_height: float, fill_level: float) -> np.ndarray:
+    """preset_blocks (src/scene/mod.rs:208-238): (n, 6) float32, min xyz then max xyz."""
+    arr = (LiquidBlock * SCENE_MAX_BLOCKS)()
+    n = C.c_uint32()
+    _check(lib().fs_scene_preset_blocks(preset, drop_height, fill_level, arr, C.byref(n)))
+    return np.array([list(arr[i].min) + list(arr[i].max) for i in range(n.value)], np.float32).reshape(-1, 6)
+
+
+def scene_from_registry(registry: "Registry") -> Scene:
+    """SceneConfig::from_settings (src/scene/mod.rs:96-116)."""
+    sc = Scene()
+    _check(lib().fs_scene_from_registry(registry._h, C.byref(sc)))
+    return sc
+
+
+def scene_seed(scene: Scene, h: float, origin, want: bool = True):
+    """generate_particles (src/gpu/fluid.rs:1838-1913) on the GPU: (k, 3)
+    float32 positions (want=False: the count k, no GPU needed)."""
+    o = (C.c_float * 3)(*origin)
+    k = C.c_uint32()
+    _check(lib().fs_scene_seed(C.byref(scene), h, o, None, 0, C.byref(k)))
+    if not want:
+        return k.value
+    out = np.zeros((k.value, 3), np.float32)
+    _check(lib().fs_scene_seed(C.byref(scene), h, o, _ptr(out), k.value, C.byref(k)))
+    return out
+
+
 class FluidStep:
     """One tank on the GPU (the GpuFluid drop-in)."""
+
+    @classmethod
+    def from_settings(cls, registry: "Registry", scene: Scene) -> "FluidStep":
+        """GpuFluid::new from a registry and a scene (fs_create_from_settings):
+        the scene's particles seeded on the GPU, at rest."""
+        h = C.c_void_p()
+        _check(lib().fs_create_from_settings(registry._h, C.byref(scene), C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.params = Params()
+        _check(lib().fs_get_params(h, C.byref(self.params)))
+        self.n = int(lib().fs_particle_count(h))
+        self.dims = (self.params.nx, self.params.ny, self.params.nz)
+        return self
 
     def __init__(self, params: Params, pos, vel=None):
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)

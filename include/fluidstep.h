@@ -256,6 +256,57 @@ FS_API int fs_forward_checkpointed(fs_tank* t, uint32_t steps, uint32_t every);
 FS_API int fs_step_adjoint(fs_tank* t, const float* gpos_final, const float* gvel_final, float* gpos_init,
                            float* gvel_init, double* gravity_grad);
 
+/* ---- scene presets and particle seeding ----
+ * The reference builds its initial particles from the registry on the host:
+ * SceneConfig::from_settings (src/scene/mod.rs:96-116) picks the preset's
+ * liquid blocks and resolves the spawn count, and generate_particles
+ * (src/gpu/fluid.rs:1833-1913) lays a jittered lattice into each block. */
+#define FS_SCENE_FALLING_BLOB 0
+#define FS_SCENE_DAM_BREAK 1
+#define FS_SCENE_DOUBLE_SPLASH 2
+#define FS_SCENE_MAX_BLOCKS 8
+
+/* LiquidBlock (src/scene/mod.rs:61-75): an AABB in normalized [0,1]^3 tank
+ * space, y up. */
+typedef struct {
+    float min[3];
+    float max[3];
+} fs_liquid_block;
+
+/* SceneConfig (src/scene/mod.rs:84-91) plus the counts derived from it. */
+typedef struct {
+    uint32_t preset;           /* FS_SCENE_* */
+    uint32_t res[3];           /* grid resolution */
+    uint32_t nblocks;
+    fs_liquid_block blocks[FS_SCENE_MAX_BLOCKS];
+    uint32_t particle_count;   /* resolved spawn target (resolved_particle_count) */
+    float effective_density;   /* effective_particle_density: particles per seeded cell */
+    uint32_t generated_count;  /* particles the lattice holds (estimated_particle_count, src/gpu/fluid.rs:1792-1831) */
+} fs_scene;
+
+/* preset_blocks (src/scene/mod.rs:208-238): the blocks of `preset` for a
+ * drop height and a fill fraction in [0,1].  *nblocks <= FS_SCENE_MAX_BLOCKS. */
+FS_API int fs_scene_preset_blocks(uint32_t preset, float drop_height, float fill_level, fs_liquid_block* blocks,
+                                  uint32_t* nblocks);
+/* SceneConfig::from_settings (src/scene/mod.rs:96-116) from the registry's
+ * scene.preset, scene.drop_height, scene.fill_level, grid.res_*,
+ * particles.density and particles.count rows. */
+FS_API int fs_scene_from_registry(const fs_registry* r, fs_scene* out);
+/* generate_particles (src/gpu/fluid.rs:1833-1913) for a tank of cell size h
+ * at `origin` (extent res * h), computed on the GPU.  *count receives the
+ * number of particles the lattice holds; pos_xyz (host, `capacity` xyz
+ * triplets) receives them, or is NULL to query the count only
+ * (FS_ERR_INVALID_ARGUMENT if capacity < *count).  Bit-identical to the
+ * reference's sequential generator. */
+FS_API int fs_scene_seed(const fs_scene* scene, float h, const float origin[3], float* pos_xyz, uint32_t capacity,
+                         uint32_t* count);
+/* GpuFluid::new (src/gpu/fluid.rs:247-318) from a registry and a scene:
+ * h = 2/64, the tank centred on the origin, the scene's particles seeded on
+ * the GPU at rest, and Params from the registry with the scene's effective
+ * density (auto rest density and surface dilation, src/scene/mod.rs:283-324).
+ * engine, precision and velocity storage as in fs_params_default. */
+FS_API int fs_create_from_settings(const fs_registry* r, const fs_scene* scene, fs_tank** out);
+
 #ifdef __cplusplus
 }
 #endif

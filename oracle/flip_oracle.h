@@ -157,6 +157,17 @@ void *orc_adj_tape_from_states_f64(const orc_params *P, uint32_t n, uint32_t ste
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 
+/* Scene presets, spawn counts and the initial lattice (flip_oracle_scene.c,
+ * src/scene/mod.rs:11-324 and src/gpu/fluid.rs:1792-1913).  Blocks are 6
+ * floats each: min xyz, max xyz in normalized tank space. */
+uint32_t orc_scene_preset_blocks(uint32_t preset, float drop_height, float fill_level, float *out);
+float orc_scene_seeded_fraction(const float *blocks, uint32_t n);
+uint32_t orc_scene_resolved_count(const uint32_t res[3], const float *blocks, uint32_t n, float density,
+                                  uint32_t override_count);
+float orc_scene_effective_density(const uint32_t res[3], const float *blocks, uint32_t n, uint32_t count);
+uint64_t orc_scene_generate(const float *blocks, uint32_t n, uint32_t particle_count, float h,
+                            const float origin[3], const uint32_t res[3], float *out, uint64_t cap, float *info);
+
 #ifdef __cplusplus
 }
 #endif

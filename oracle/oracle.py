@@ -405,3 +405,68 @@ def run_frozen(P, pos, vel, ct_steps, occ_steps, pressure=None, f64=True):
                                             vp(*[ptr(c) for c in cols_v]), ptr(pr), C.c_uint32(len(ct)), ptr(ct),
                                             ptr(occ))
     return np.stack(cols_p, 1), np.stack(cols_v, 1), pr
+
+
+# ---------------------------------------------------------------------------
+# Scene presets, spawn counts and the initial lattice (flip_oracle_scene.c).
+
+def _scene_lib():
+    L = lib()
+    if not getattr(L, "_scene_typed", False):
+        L.orc_scene_preset_blocks.restype = C.c_uint32
+        L.orc_scene_preset_blocks.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_void_p]
+        L.orc_scene_seeded_fraction.restype = C.c_float
+        L.orc_scene_seeded_fraction.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_scene_resolved_count.restype = C.c_uint32
+        L.orc_scene_resolved_count.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32]
+        L.orc_scene_effective_density.restype = C.c_float
+        L.orc_scene_effective_density.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.orc_scene_generate.restype = C.c_uint64
+        L.orc_scene_generate.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_uint64, C.c_void_p]
+        L._scene_typed = True
+    return L
+
+
+def scene_preset_blocks(preset, drop_height, fill_level) -> np.ndarray:
+    """preset_blocks (src/scene/mod.rs:208-238): (n, 6) float32, min xyz then max xyz."""
+    out = np.zeros((8, 6), np.float32)
+    n = _scene_lib().orc_scene_preset_blocks(preset, np.float32(drop_height), np.float32(fill_level), ptr(out))
+    return out[:n].copy()
+
+
+def scene_seeded_fraction(blocks) -> np.float32:
+    """seeded_volume_fraction (src/scene/mod.rs:134-146)."""
+    b = np.ascontiguousarray(blocks, np.float32)
+    return np.float32(_scene_lib().orc_scene_seeded_fraction(ptr(b), len(b)))
+
+
+def scene_resolved_count(res, blocks, density, override=0) -> int:
+    """resolved_particle_count (src/scene/mod.rs:179-193)."""
+    b = np.ascontiguousarray(blocks, np.float32)
+    r = np.asarray(res, np.uint32)
+    return int(_scene_lib().orc_scene_resolved_count(ptr(r), ptr(b), len(b), np.float32(density), override))
+
+
+def scene_effective_density(res, blocks, count) -> np.float32:
+    """effective_particle_density_for_count (src/scene/mod.rs:278-289)."""
+    b = np.ascontiguousarray(blocks, np.float32)
+    r = np.asarray(res, np.uint32)
+    return np.float32(_scene_lib().orc_scene_effective_density(ptr(r), ptr(b), len(b), count))
+
+
+def scene_generate(blocks, particle_count, h, origin, res, want=True, info=False):
+    """generate_particles (src/gpu/fluid.rs:1838-1913): (k, 3) float32 positions
+    (want=False: the count k only).  info=True also returns the per-block
+    lattice as an (n, 7) array: wmin xyz, spacing, counts xyz."""
+    b = np.ascontiguousarray(blocks, np.float32)
+    r = np.asarray(res, np.uint32)
+    o = np.asarray(origin, np.float32)
+    L = _scene_lib()
+    inf = np.zeros((max(len(b), 1), 7), np.float32)
+    k = int(L.orc_scene_generate(ptr(b), len(b), particle_count, np.float32(h), ptr(o), ptr(r), None, 0, ptr(inf)))
+    if not want:
+        return (k, inf[:len(b)]) if info else k
+    out = np.zeros((k, 3), np.float32)
+    L.orc_scene_generate(ptr(b), len(b), particle_count, np.float32(h), ptr(o), ptr(r), ptr(out), k, None)
+    return (out, inf[:len(b)]) if info else out
