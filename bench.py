@@ -365,6 +365,46 @@ def precision_report(steps, teacher_states):
     return report
 
 
+def scene_report(fsm, reps=5):
+    """Initial particles of a scene (generate_particles, src/gpu/fluid.rs:
+    1838-1913): fs_scene_seed on the GPU (device buffer, kernel, copy to the
+    host) against the sequential CPU restatement, per case, with the bits
+    compared.  The seeding kernel's own duration is in the rocprof trace of
+    this command."""
+    import oracle
+    H = np.float32(2.0 / 64.0)
+    cases = [
+        ("default_falling_blob_80x40x80_d10", dict()),
+        ("dam_break_80x40x80_fill50_d8", {"scene.preset": 1, "scene.fill_level": 50.0, "particles.density": 8.0}),
+        ("falling_blob_128^3_fill50_d32", {"grid.res_x": 128, "grid.res_y": 128, "grid.res_z": 128,
+                                           "scene.fill_level": 50.0, "particles.density": 32.0}),
+    ]
+    out = {"report": "scene_seed", "cases": []}
+    for name, settings in cases:
+        r = fsm.Registry()
+        for k, v in settings.items():
+            r.set(k, v)
+        sc = fsm.scene_from_registry(r)
+        origin = [-(np.float32(n) * H) / np.float32(2.0) for n in sc.res]
+        pos = fsm.scene_seed(sc, H, origin)  # warm-up (module load, first allocation)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            pos = fsm.scene_seed(sc, H, origin)
+            times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ref = oracle.scene_generate(sc.blocks_array(), sc.particle_count, H, origin, tuple(sc.res))
+        cpu_s = time.perf_counter() - t0
+        gpu_s = float(np.median(times))
+        ent = {"case": name, "particles": int(len(pos)), "target": int(sc.particle_count),
+               "gpu_ms_per_call": gpu_s * 1e3, "gpu_particles_per_s": len(pos) / gpu_s,
+               "cpu_ms": cpu_s * 1e3, "cpu_particles_per_s": len(pos) / cpu_s, "cpu_cores": 1,
+               "bit_exact": bool(pos.shape == ref.shape and np.array_equal(pos.view(np.uint32), ref.view(np.uint32)))}
+        out["cases"].append(ent)
+        print(name, json.dumps(ent), file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +422,8 @@ def main():
     ap.add_argument("--adjoint-report", type=int, default=0, metavar="STEPS",
                     help="instead of timing, run the differentiable-control check at 64^3 over STEPS substeps")
     ap.add_argument("--no-fp64", action="store_true", help="adjoint report: skip the FP64 reference gradient")
+    ap.add_argument("--scene-report", action="store_true",
+                    help="instead of timing the step, time scene seeding on the GPU against the CPU restatement")
     ap.add_argument("--workload", default="high_density_128",
                     choices=["high_density_128", "large_splash_256", "dam_break_32", "control_64", "batched_tanks"],
                     help="the metric is quoted on high_density_128; the others are for per-kernel profiles "
@@ -409,6 +451,10 @@ def main():
     if args.adjoint_report:
         print(json.dumps(adjoint_report(fsm, scenes, gpu_params, steps=args.adjoint_report, fp64=not args.no_fp64)),
               flush=True)
+        return
+
+    if args.scene_report:
+        print(json.dumps(scene_report(fsm)), flush=True)
         return
 
     if args.workload == "batched_tanks":

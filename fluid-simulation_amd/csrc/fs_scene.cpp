@@ -172,6 +172,8 @@ int build_layout(const fs_scene& sc, float h, const float origin[3], SeedLayout&
         }
         B.spacing = spacing;
         B.first = (uint32_t)first;
+        magic_u31(B.counts[2], B.mz, B.sz);
+        magic_u31(B.counts[1], B.my, B.sy);
         first += cnt;
         // Particle g draws the generator's states 3g+1 .. 3g+3 (32-bit).
         if (first > 0x55555555ull) return fail(FS_ERR_INVALID_ARGUMENT, "scene lattice too large");
