@@ -405,6 +405,38 @@ def scene_report(fsm, reps=5):
     return out
 
 
+def view_report(fsm, scenes, gpu_params, reps=20):
+    """The live-buffer hand-off on the 128^3 tank: fs_get_device_view (no
+    copy) and fs_pack_particles_vec4, the reference's 32-byte particle record
+    packed from the live side (24 B read + 32 B written per particle).  The
+    pack kernel's own duration is in the rocprof trace of this command."""
+    import torch
+    sc = scenes.high_density_128()
+    t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED), sc["pos"], sc["vel"])
+    t.step(1)
+    t.synchronize()
+    n = t.n
+    dst = torch.empty(8 * n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t.device_view()
+    view_s = (time.perf_counter() - t0) / reps
+    t.pack_particles_vec4(dst.data_ptr())
+    t.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t.pack_particles_vec4(dst.data_ptr())
+    t.synchronize()
+    pack_s = (time.perf_counter() - t0) / reps
+    t.close()
+    return {"report": "device_view", "workload": sc["name"], "particles": n,
+            "device_view_us_per_call": view_s * 1e6,
+            "pack_vec4_ms_per_call": pack_s * 1e3,
+            "pack_vec4_algorithmic_GBps": 56.0 * n / pack_s / 1e9,
+            "note": "pack timed as back-to-back calls on the tank's stream, host wall clock"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +454,8 @@ def main():
     ap.add_argument("--adjoint-report", type=int, default=0, metavar="STEPS",
                     help="instead of timing, run the differentiable-control check at 64^3 over STEPS substeps")
     ap.add_argument("--no-fp64", action="store_true", help="adjoint report: skip the FP64 reference gradient")
+    ap.add_argument("--view-report", action="store_true",
+                    help="instead of timing the step, time the live-buffer hand-off and the vec4 particle pack")
     ap.add_argument("--scene-report", action="store_true",
                     help="instead of timing the step, time scene seeding on the GPU against the CPU restatement")
     ap.add_argument("--workload", default="high_density_128",
@@ -455,6 +489,10 @@ def main():
 
     if args.scene_report:
         print(json.dumps(scene_report(fsm)), flush=True)
+        return
+
+    if args.view_report:
+        print(json.dumps(view_report(fsm, scenes, gpu_params)), flush=True)
         return
 
     if args.workload == "batched_tanks":

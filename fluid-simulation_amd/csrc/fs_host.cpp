@@ -273,6 +273,7 @@ int sort_particles(fs_tank* t) {
     FS_HIP(launch_stable_permutation(t->keys, P.N, P.C, t->ktmp0, t->ktmp1, t->vtmp, t->perm, t->radix_ws, s));
     FS_HIP(launch_permute(P.N, t->perm, t->side[t->cur], t->side[t->cur ^ 1], t->bf16, s));
     t->cur ^= 1;
+    t->sort_swapped = true;  // src/gpu/fluid.rs:1570
     t->perm_valid = 1;
     return FS_OK;
 }
@@ -686,6 +687,7 @@ FS_API int fs_reset(fs_tank* t) {
     t->tick = 0;
     t->substeps = 0;
     t->perm_valid = 0;
+    t->sort_swapped = true;  // src/gpu/fluid.rs:1415
     return FS_OK;
 }
 
@@ -699,6 +701,7 @@ FS_API int fs_load_state(fs_tank* t, const float* pos_xyz, const float* vel_xyz,
     if (t->N > 0) {
         int rc = upload_particles(t, pos_xyz, vel_xyz);
         if (rc != FS_OK) return rc;
+        t->sort_swapped = true;  // the live side is side 0 again
     }
     if (pressure) {
         FS_HIP(hipMemcpyAsync(t->p, pressure, sizeof(float) * t->P.C, hipMemcpyHostToDevice, t->stream));
@@ -710,6 +713,53 @@ FS_API int fs_load_state(fs_tank* t, const float* pos_xyz, const float* vel_xyz,
 FS_API int fs_step(fs_tank* t, uint32_t substeps) {
     if (int rc = check_tank(t)) return rc;
     return do_step(t, substeps);
+}
+
+FS_API int fs_get_device_view(fs_tank* t, fs_device_view* out) {
+    if (int rc = check_tank(t)) return rc;
+    if (!out) return fail(FS_ERR_INVALID_ARGUMENT, "null view");
+    fs_device_view v;
+    std::memset(&v, 0, sizeof v);
+    const ParticleSoA& sd = t->side[t->cur];
+    for (int a = 0; a < 3; ++a) {
+        v.pos[a] = sd.pos[a];
+        v.vel[a] = sd.vel[a];
+        v.face_vel[a] = t->vel[a];
+    }
+    v.n = t->N;
+    v.velocity_storage = t->bf16 ? FS_VELOCITY_BF16 : FS_VELOCITY_F32;
+    v.live_side = (uint32_t)t->cur;
+    v.dims[0] = t->P.nx;
+    v.dims[1] = t->P.ny;
+    v.dims[2] = t->P.nz;
+    v.cell_type = t->ct;
+    v.pressure = t->p;
+    v.occupancy = t->occ;
+    v.stats = t->counters;
+    v.stream = (void*)t->stream;
+    *out = v;
+    return FS_OK;
+}
+
+FS_API int fs_take_sort_swapped(fs_tank* t) {
+    if (check_tank(t) != FS_OK) return 0;
+    const bool s = t->sort_swapped;
+    t->sort_swapped = false;
+    return s ? 1 : 0;
+}
+
+FS_API int fs_pack_particles_vec4(fs_tank* t, float* dst_device) {
+    if (int rc = check_tank(t)) return rc;
+    if (!dst_device && t->N > 0) return fail(FS_ERR_INVALID_ARGUMENT, "null destination");
+    FS_HIP(launch_pack_particles_vec4(t->side[t->cur], t->N, t->bf16, dst_device, t->stream));
+    return FS_OK;
+}
+
+FS_API int fs_pack_cell_types_u32(fs_tank* t, uint32_t* dst_device) {
+    if (int rc = check_tank(t)) return rc;
+    if (!dst_device) return fail(FS_ERR_INVALID_ARGUMENT, "null destination");
+    FS_HIP(launch_widen_u8(t->ct, t->P.C, dst_device, t->stream));
+    return FS_OK;
 }
 
 FS_API int fs_synchronize(fs_tank* t) {

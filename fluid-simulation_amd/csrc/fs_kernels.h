@@ -174,6 +174,10 @@ hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* 
 // ---- k_g2p.hip ----
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16,
                       const float* const vel[3], const float* const saved[3], hipStream_t s);
+// The reference's 32-byte particle record {pos.xyz, 0, vel.xyz, 0} of the
+// side `sp` into dst (8n floats), and u8 -> u32 widening (k_view.hip).
+hipError_t launch_pack_particles_vec4(const ParticleSoA& sp, uint32_t n, bool bf16, float* dst, hipStream_t s);
+hipError_t launch_widen_u8(const uint8_t* src, uint32_t n, uint32_t* dst, hipStream_t s);
 hipError_t launch_impulse(const StepParams& P, const ParticleSoA& live, int bf16, float dx, float dy,
                           float dz, hipStream_t s);
 

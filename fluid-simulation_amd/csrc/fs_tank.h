@@ -28,6 +28,7 @@ struct fs_tank {
 
     ParticleSoA side[2]{};
     int cur = 0;
+    bool sort_swapped = false;  // take_sort_swapped: the live side flipped since the last query
     uint32_t *keys = nullptr, *ktmp0 = nullptr, *ktmp1 = nullptr, *vtmp = nullptr, *perm = nullptr;
     uint32_t *radix_ws = nullptr, *scan_ws = nullptr;
     uint32_t *occ = nullptr, *cell_offset = nullptr;

@@ -44,6 +44,7 @@ EXPORTED = (
     "fs_batch_read_cells_f32", "fs_batch_read_cells_u32", "fs_batch_read_permutation",
     "fs_batch_get_stats", "fs_forward_checkpointed", "fs_step_adjoint",
     "fs_scene_preset_blocks", "fs_scene_from_registry", "fs_scene_seed", "fs_create_from_settings",
+    "fs_get_device_view", "fs_take_sort_swapped", "fs_pack_particles_vec4", "fs_pack_cell_types_u32",
 )
 
 
@@ -123,6 +124,15 @@ class Tuning(C.Structure):
             "cg_unroll": self.CG_UNROLL[self.cg_unroll], "cg_blocks": self.cg_blocks,
             "ms_gather": self.ms_gather, "ms_sort": self.ms_sort, "ms_cg": self.ms_cg,
         }
+
+
+class DeviceView(C.Structure):
+    """fs_device_view: device pointers of a tank's live state."""
+    _fields_ = [
+        ("pos", C.c_void_p * 3), ("vel", C.c_void_p * 3), ("n", C.c_uint32), ("velocity_storage", C.c_uint32),
+        ("live_side", C.c_uint32), ("dims", C.c_uint32 * 3), ("cell_type", C.c_void_p), ("pressure", C.c_void_p),
+        ("face_vel", C.c_void_p * 3), ("occupancy", C.c_void_p), ("stats", C.c_void_p), ("stream", C.c_void_p),
+    ]
 
 
 class LiquidBlock(C.Structure):
@@ -212,6 +222,10 @@ def lib():
         L.fs_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
         L.fs_forward_checkpointed.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.fs_step_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.fs_get_device_view.argtypes = [vp, C.POINTER(DeviceView)]
+        L.fs_take_sort_swapped.argtypes = [vp]
+        L.fs_pack_particles_vec4.argtypes = [vp, vp]
+        L.fs_pack_cell_types_u32.argtypes = [vp, vp]
         L.fs_scene_preset_blocks.argtypes = [C.c_uint32, C.c_float, C.c_float, C.POINTER(LiquidBlock),
                                              C.POINTER(C.c_uint32)]
         L.fs_scene_from_registry.argtypes = [vp, C.POINTER(Scene)]
@@ -363,6 +377,22 @@ class FluidStep:
 
     def reset(self):
         _check(lib().fs_reset(self._h))
+
+    # live device buffers
+    def device_view(self) -> DeviceView:
+        v = DeviceView()
+        _check(lib().fs_get_device_view(self._h, C.byref(v)))
+        return v
+
+    def take_sort_swapped(self) -> bool:
+        return bool(lib().fs_take_sort_swapped(self._h))
+
+    def pack_particles_vec4(self, dst_device_ptr: int):
+        """{pos.xyz, 0, vel.xyz, 0} per particle into device memory (8n floats)."""
+        _check(lib().fs_pack_particles_vec4(self._h, C.c_void_p(dst_device_ptr)))
+
+    def pack_cell_types_u32(self, dst_device_ptr: int):
+        _check(lib().fs_pack_cell_types_u32(self._h, C.c_void_p(dst_device_ptr)))
 
     def load_state(self, pos, vel=None, pressure=None):
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)

@@ -256,6 +256,44 @@ FS_API int fs_forward_checkpointed(fs_tank* t, uint32_t steps, uint32_t every);
 FS_API int fs_step_adjoint(fs_tank* t, const float* gpos_final, const float* gvel_final, float* gpos_init,
                            float* gvel_init, double* gravity_grad);
 
+/* ---- live device buffers (zero-copy hand-off to a consumer on the GPU) ----
+ * The reference lends its buffers to the renderer and slice inspector:
+ * particle_buffer() (the live ping-pong side), take_sort_swapped(),
+ * cell_type_buffer(), pressure_buffer(), u/v/w_vel_buffer(), stats_buffer()
+ * (src/gpu/fluid.rs:1205-1245); the consumer rebinds when the live side
+ * flips (src/gpu/mod.rs:983-988).  Here the same state is lent as device
+ * pointers in this library's layouts.  All of the tank's work is ordered on
+ * its HIP stream (`stream`): a consumer enqueues its reads on that stream or
+ * waits for it (fs_synchronize).  Pointers stay valid until fs_destroy; the
+ * particle pointers name the live side until the next fs_step or fs_reset,
+ * which may flip it (then fs_take_sort_swapped reports 1). */
+typedef struct {
+    const float* pos[3];        /* live particle side, SoA: n floats each */
+    const void* vel[3];         /* n floats, or n bf16 as uint16 (FS_VELOCITY_BF16) */
+    uint32_t n;
+    uint32_t velocity_storage;  /* FS_VELOCITY_* */
+    uint32_t live_side;         /* 0 or 1 */
+    uint32_t dims[3];           /* nx, ny, nz */
+    const uint8_t* cell_type;   /* nx*ny*nz: 0 solid, 1 liquid, 2 air (the reference stores u32) */
+    const float* pressure;      /* nx*ny*nz */
+    const float* face_vel[3];   /* (nx+1)ny nz, nx(ny+1)nz, nx ny(nz+1) */
+    const uint32_t* occupancy;  /* nx*ny*nz particles per cell */
+    const uint32_t* stats;      /* stats[0] = liquid cell count (sh/classify.wgsl:55-57) */
+    void* stream;               /* the tank's hipStream_t */
+} fs_device_view;
+
+FS_API int fs_get_device_view(fs_tank* t, fs_device_view* out);
+/* take_sort_swapped (src/gpu/fluid.rs:1219-1226): 1 if the live particle
+ * side flipped (a sort, a reset or a state load) since the last call, then 0. */
+FS_API int fs_take_sort_swapped(fs_tank* t);
+/* The reference's particle record {pos.xyz, 0, vel.xyz, 0} (32 B,
+ * src/gpu/fluid.rs:276-280, read by sh/particles.wgsl:22) for the live side,
+ * written to dst (device, 8n floats), enqueued on the tank's stream. */
+FS_API int fs_pack_particles_vec4(fs_tank* t, float* dst_device);
+/* Cell types as u32, the reference's cell_type buffer, written to dst
+ * (device, nx*ny*nz), enqueued on the tank's stream. */
+FS_API int fs_pack_cell_types_u32(fs_tank* t, uint32_t* dst_device);
+
 /* ---- scene presets and particle seeding ----
  * The reference builds its initial particles from the registry on the host:
  * SceneConfig::from_settings (src/scene/mod.rs:96-116) picks the preset's
