@@ -62,6 +62,12 @@ const Row kRows[] = {
     {"physics.volume_stiffness", false, 0.45, 0.0, 4.0, FS_APPLY_LIVE},
     {"physics.drift_clamp", false, 0.5, 0.05, 2.0, FS_APPLY_LIVE},
     {"physics.cfl", false, 2.0, 1.0, 6.0, FS_APPLY_LIVE},
+    {"interaction.auto_roll_enabled", true, 0, 0, 1, FS_APPLY_LIVE},
+    {"interaction.auto_roll_strength", false, 0.222, 0.0, 1.2, FS_APPLY_LIVE},
+    {"interaction.auto_roll_cadence", false, 0.8375, 0.5, 8.0, FS_APPLY_LIVE},
+    {"interaction.wave_enabled", true, 0, 0, 1, FS_APPLY_LIVE},
+    {"interaction.wave_strength", false, 2.145, 0.0, 3.0, FS_APPLY_LIVE},
+    {"interaction.wave_frequency", false, 0.22775, 0.05, 4.0, FS_APPLY_LIVE},
     {"classify.liquid_threshold", true, 1, 1, 8, FS_APPLY_LIVE},
     {"classify.surface_dilation", true, 0, 0, 1, FS_APPLY_LIVE},
     {"solver.pressure_iterations", true, 30, 1, 200, FS_APPLY_LIVE},

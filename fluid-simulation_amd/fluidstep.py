@@ -45,6 +45,12 @@ EXPORTED = (
     "fs_batch_get_stats", "fs_forward_checkpointed", "fs_step_adjoint",
     "fs_scene_preset_blocks", "fs_scene_from_registry", "fs_scene_seed", "fs_create_from_settings",
     "fs_get_device_view", "fs_take_sort_swapped", "fs_pack_particles_vec4", "fs_pack_cell_types_u32",
+    "fs_timestep_create", "fs_timestep_destroy", "fs_timestep_steps_for_frame", "fs_timestep_record_manual",
+    "fs_timestep_record_export", "fs_timestep_record_idle", "fs_timestep_reset", "fs_timestep_last_stats",
+    "fs_timestep_total_dropped", "fs_interactions_create", "fs_interactions_destroy", "fs_interactions_reset",
+    "fs_interactions_auto_roll", "fs_interactions_wave", "fs_interactions_next_u32", "fs_frame_create",
+    "fs_frame_destroy", "fs_frame_run", "fs_frame_export", "fs_frame_set_paused", "fs_frame_queue_step",
+    "fs_frame_reset", "fs_frame_tick", "fs_frame_orientation",
 )
 
 
@@ -124,6 +130,20 @@ class Tuning(C.Structure):
             "cg_unroll": self.CG_UNROLL[self.cg_unroll], "cg_blocks": self.cg_blocks,
             "ms_gather": self.ms_gather, "ms_sort": self.ms_sort, "ms_cg": self.ms_cg,
         }
+
+
+class TimestepStats(C.Structure):
+    """fs_timestep_stats: TimestepFrameStats (src/timestep.rs:20-49)."""
+    _fields_ = [
+        ("substeps", C.c_uint32), ("natural_substeps", C.c_uint32), ("max_substeps", C.c_uint32),
+        ("substep_cap_hit", C.c_uint32), ("fixed_dt", C.c_float), ("accumulated_before", C.c_float),
+        ("accumulated_after", C.c_float), ("dropped_this_frame", C.c_float), ("sim_advanced", C.c_float),
+        ("wall_dt", C.c_float), ("sim_time_scale", C.c_float), ("real_time_factor", C.c_float),
+        ("policy", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class DeviceView(C.Structure):
@@ -222,6 +242,45 @@ def lib():
         L.fs_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
         L.fs_forward_checkpointed.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.fs_step_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.fs_timestep_create.argtypes = [C.c_float, C.c_uint32]
+        L.fs_timestep_create.restype = vp
+        L.fs_timestep_destroy.argtypes = [vp]
+        L.fs_timestep_destroy.restype = None
+        L.fs_timestep_steps_for_frame.argtypes = [vp, C.c_float, C.c_float]
+        L.fs_timestep_steps_for_frame.restype = C.c_uint32
+        L.fs_timestep_record_manual.argtypes = [vp, C.c_float, C.c_uint32, C.c_float]
+        L.fs_timestep_record_manual.restype = None
+        L.fs_timestep_record_export.argtypes = [vp, C.c_float, C.c_uint32]
+        L.fs_timestep_record_export.restype = None
+        L.fs_timestep_record_idle.argtypes = [vp, C.c_float, C.c_float]
+        L.fs_timestep_record_idle.restype = None
+        L.fs_timestep_reset.argtypes = [vp]
+        L.fs_timestep_reset.restype = None
+        L.fs_timestep_last_stats.argtypes = [vp, C.POINTER(TimestepStats)]
+        L.fs_timestep_total_dropped.argtypes = [vp]
+        L.fs_timestep_total_dropped.restype = C.c_float
+        L.fs_interactions_create.argtypes = [C.c_uint64]
+        L.fs_interactions_create.restype = vp
+        L.fs_interactions_destroy.argtypes = [vp]
+        L.fs_interactions_destroy.restype = None
+        L.fs_interactions_reset.argtypes = [vp, C.c_uint64]
+        L.fs_interactions_reset.restype = None
+        L.fs_interactions_auto_roll.argtypes = [vp, C.c_float, C.POINTER(C.c_float), C.c_int, C.c_float, C.c_float,
+                                                C.POINTER(C.c_float)]
+        L.fs_interactions_wave.argtypes = [vp, C.c_float, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float)]
+        L.fs_interactions_next_u32.argtypes = [vp]
+        L.fs_interactions_next_u32.restype = C.c_uint32
+        L.fs_frame_create.argtypes = [vp, vp, C.POINTER(vp)]
+        L.fs_frame_destroy.argtypes = [vp]
+        L.fs_frame_destroy.restype = None
+        L.fs_frame_run.argtypes = [vp, C.c_double, C.POINTER(TimestepStats)]
+        L.fs_frame_export.argtypes = [vp, C.c_uint32, C.c_double, C.POINTER(TimestepStats)]
+        L.fs_frame_set_paused.argtypes = [vp, C.c_int]
+        L.fs_frame_queue_step.argtypes = [vp]
+        L.fs_frame_reset.argtypes = [vp]
+        L.fs_frame_tick.argtypes = [vp]
+        L.fs_frame_tick.restype = C.c_uint64
+        L.fs_frame_orientation.argtypes = [vp, C.POINTER(C.c_float)]
         L.fs_get_device_view.argtypes = [vp, C.POINTER(DeviceView)]
         L.fs_take_sort_swapped.argtypes = [vp]
         L.fs_pack_particles_vec4.argtypes = [vp, vp]
@@ -298,6 +357,116 @@ class Registry:
         o = (C.c_float * 3)(*origin)
         _check(lib().fs_registry_to_params(self._h, h, o, scene_density, C.byref(p)))
         return p
+
+
+class Timestep:
+    """TimestepController (src/timestep.rs:63-247), host only."""
+
+    def __init__(self, fixed_dt: float, max_substeps: int):
+        self._h = lib().fs_timestep_create(fixed_dt, max_substeps)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fs_timestep_destroy(self._h)
+            self._h = None
+
+    def steps_for_frame(self, render_dt_s: float, scale: float = 1.0) -> int:
+        return int(lib().fs_timestep_steps_for_frame(self._h, render_dt_s, scale))
+
+    def record_manual(self, render_dt_s, substeps, scale):
+        lib().fs_timestep_record_manual(self._h, render_dt_s, substeps, scale)
+
+    def record_export(self, sim_dt_s, substeps):
+        lib().fs_timestep_record_export(self._h, sim_dt_s, substeps)
+
+    def record_idle(self, render_dt_s, scale):
+        lib().fs_timestep_record_idle(self._h, render_dt_s, scale)
+
+    def reset(self):
+        lib().fs_timestep_reset(self._h)
+
+    def last_stats(self) -> dict:
+        st = TimestepStats()
+        _check(lib().fs_timestep_last_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def total_dropped(self) -> float:
+        return float(lib().fs_timestep_total_dropped(self._h))
+
+
+class Interactions:
+    """InteractionState (src/lib.rs:55-236), host only."""
+
+    def __init__(self, seed: int):
+        self._h = lib().fs_interactions_create(seed)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fs_interactions_destroy(self._h)
+            self._h = None
+
+    def reset(self, seed: int):
+        lib().fs_interactions_reset(self._h, seed)
+
+    def next_u32(self) -> int:
+        return int(lib().fs_interactions_next_u32(self._h))
+
+    def auto_roll(self, dt, current, enabled, strength, cadence):
+        cur = (C.c_float * 4)(*current)
+        out = (C.c_float * 4)()
+        if not lib().fs_interactions_auto_roll(self._h, dt, cur, int(enabled), strength, cadence, out):
+            return None
+        return np.array(list(out), np.float32)
+
+    def wave(self, dt, enabled, strength, frequency):
+        out = (C.c_float * 3)()
+        if not lib().fs_interactions_wave(self._h, dt, int(enabled), strength, frequency, out):
+            return None
+        return np.array(list(out), np.float32)
+
+
+class Frame:
+    """FluidApp's frame loop around one tank (fs_frame_*); `settings` is read
+    at every call and must outlive the driver."""
+
+    def __init__(self, tank: "FluidStep", settings: "Registry"):
+        h = C.c_void_p()
+        _check(lib().fs_frame_create(tank._h, settings._h, C.byref(h)))
+        self._h, self.tank, self.settings = h, tank, settings
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fs_frame_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def run(self, render_dt_ms: float) -> dict:
+        st = TimestepStats()
+        _check(lib().fs_frame_run(self._h, render_dt_ms, C.byref(st)))
+        return st.as_dict()
+
+    def export(self, substeps: int, sim_dt_s: float) -> dict:
+        st = TimestepStats()
+        _check(lib().fs_frame_export(self._h, substeps, sim_dt_s, C.byref(st)))
+        return st.as_dict()
+
+    def set_paused(self, paused: bool):
+        _check(lib().fs_frame_set_paused(self._h, int(paused)))
+
+    def queue_step(self):
+        _check(lib().fs_frame_queue_step(self._h))
+
+    def reset(self):
+        _check(lib().fs_frame_reset(self._h))
+
+    def tick(self) -> int:
+        return int(lib().fs_frame_tick(self._h))
+
+    def orientation(self) -> np.ndarray:
+        q = (C.c_float * 4)()
+        _check(lib().fs_frame_orientation(self._h, q))
+        return np.array(list(q), np.float32)
 
 
 def scene_preset_blocks(preset: int, drop_height: float, fill_level: float) -> np.ndarray:
@@ -377,6 +546,12 @@ class FluidStep:
 
     def reset(self):
         _check(lib().fs_reset(self._h))
+
+    def current_params(self) -> Params:
+        """The tank's parameters now (live setters and fs_set_gravity included)."""
+        p = Params()
+        _check(lib().fs_get_params(self._h, C.byref(p)))
+        return p
 
     # live device buffers
     def device_view(self) -> DeviceView:

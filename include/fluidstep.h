@@ -294,6 +294,75 @@ FS_API int fs_pack_particles_vec4(fs_tank* t, float* dst_device);
  * (device, nx*ny*nz), enqueued on the tank's stream. */
 FS_API int fs_pack_cell_types_u32(fs_tank* t, uint32_t* dst_device);
 
+/* ---- frame driver: the host loop around fs_step ----
+ * TimestepController (src/timestep.rs:63-247): a fixed-dt accumulator fed by
+ * render deltas, each clamped to 1/30 s, running at most max_substeps per
+ * frame and dropping the excess when capped.  Host only. */
+#define FS_TIMESTEP_FIXED_DROP_CAP 0        /* "fixed-drop-cap" (TIMESTEP_POLICY_LABEL) */
+#define FS_TIMESTEP_FIXED_EXPLICIT_EXPORT 1 /* "fixed-explicit-export" */
+/* TimestepFrameStats (src/timestep.rs:20-49). */
+typedef struct {
+    uint32_t substeps, natural_substeps, max_substeps, substep_cap_hit;
+    float fixed_dt, accumulated_before, accumulated_after, dropped_this_frame, sim_advanced, wall_dt,
+        sim_time_scale, real_time_factor;
+    uint32_t policy; /* FS_TIMESTEP_* */
+} fs_timestep_stats;
+typedef struct fs_timestep fs_timestep;
+FS_API fs_timestep* fs_timestep_create(float fixed_dt, uint32_t max_substeps); /* TimestepController::new */
+FS_API void fs_timestep_destroy(fs_timestep* c);
+/* steps_for_frame_scaled (:104-159): substeps to run for this frame. */
+FS_API uint32_t fs_timestep_steps_for_frame(fs_timestep* c, float render_dt_s, float sim_time_scale);
+FS_API void fs_timestep_record_manual(fs_timestep* c, float render_dt_s, uint32_t substeps, float sim_time_scale);
+FS_API void fs_timestep_record_export(fs_timestep* c, float sim_dt_s, uint32_t substeps);
+FS_API void fs_timestep_record_idle(fs_timestep* c, float render_dt_s, float sim_time_scale);
+FS_API void fs_timestep_reset(fs_timestep* c);
+FS_API int fs_timestep_last_stats(const fs_timestep* c, fs_timestep_stats* out);
+FS_API float fs_timestep_total_dropped(const fs_timestep* c);
+
+/* Interaction schedules (InteractionState, src/lib.rs:55-264): the seeded
+ * auto-roll of the tank (a rotation that eases to a new random target every
+ * ~cadence seconds, at most `strength` radians from upright) and the periodic
+ * wave impulse (+x, -x, +z, -z in turn).  Host only.  Quaternions are
+ * x, y, z, w. */
+typedef struct fs_interactions fs_interactions;
+FS_API fs_interactions* fs_interactions_create(uint64_t seed); /* InteractionState::new */
+FS_API void fs_interactions_destroy(fs_interactions* s);
+FS_API void fs_interactions_reset(fs_interactions* s, uint64_t seed);
+/* update_auto_roll (:146-186): 1 and the new pose in q_out, or 0 (disabled). */
+FS_API int fs_interactions_auto_roll(fs_interactions* s, float dt_s, const float q_current[4], int enabled,
+                                     float strength, float cadence_s, float q_out[4]);
+/* update_wave (:205-236): 1 and the impulse (velocity change) in dv_out, or 0. */
+FS_API int fs_interactions_wave(fs_interactions* s, float dt_s, int enabled, float strength, float frequency_hz,
+                                float dv_out[3]);
+/* InteractionRng::next_u32 (:70-76), for checking the generator. */
+FS_API uint32_t fs_interactions_next_u32(fs_interactions* s);
+
+/* FluidApp's frame loop around one tank (src/lib.rs:352-398, 570-601, 506-560,
+ * 907-937): run state, the timestep controller, the interactions and the
+ * tank's gravity in the box frame (box_orient^-1 * (0, physics.gravity, 0)).
+ * Settings are read from `settings` at every call (the caller's registry,
+ * which must outlive the driver): physics.fixed_dt and physics.max_substeps
+ * at create and reset, physics.sim_speed and interaction.* per frame. */
+typedef struct fs_frame fs_frame;
+FS_API int fs_frame_create(fs_tank* t, const fs_registry* settings, fs_frame** out);
+FS_API void fs_frame_destroy(fs_frame* f);
+/* FluidApp::frame: Running runs the accumulator's substeps (after the
+ * interactions, at min(render_dt * sim_speed, 1/30 s)); Paused runs one
+ * queued step or none.  render_dt_ms: the browser rAF delta (non-finite or
+ * negative reads as 0). */
+FS_API int fs_frame_run(fs_frame* f, double render_dt_ms, fs_timestep_stats* out);
+/* FluidApp::export_frame (:570-601): exactly `substeps` substeps for
+ * sim_dt_s seconds of simulation. */
+FS_API int fs_frame_export(fs_frame* f, uint32_t substeps, double sim_dt_s, fs_timestep_stats* out);
+FS_API int fs_frame_set_paused(fs_frame* f, int paused);
+FS_API int fs_frame_queue_step(fs_frame* f); /* the step button while paused */
+/* FluidApp::reset (:520-560) for the tank: fs_reset, tick and queue cleared,
+ * a new timestep controller from the settings, upright box, interactions
+ * reseeded (interaction_seed_for_reset, :992-994), gravity pushed. */
+FS_API int fs_frame_reset(fs_frame* f);
+FS_API uint64_t fs_frame_tick(const fs_frame* f);
+FS_API int fs_frame_orientation(const fs_frame* f, float q_out[4]); /* box_orient */
+
 /* ---- scene presets and particle seeding ----
  * The reference builds its initial particles from the registry on the host:
  * SceneConfig::from_settings (src/scene/mod.rs:96-116) picks the preset's
