@@ -437,6 +437,32 @@ def view_report(fsm, scenes, gpu_params, reps=20):
             "note": "pack timed as back-to-back calls on the tank's stream, host wall clock"}
 
 
+def sections_report(fsm, scenes, gpu_params, steps=2):
+    """Per-section device time of both engines on the 128^3 tank under the
+    reference's FINE_SECTIONS / CG_CATS names (fs_get_detailed_timing), each
+    engine's coarse spans from a run without the detailed events beside it,
+    its kernel dispatches per substep, and fs_stats_json's object."""
+    sc = scenes.high_density_128()
+    out = {"report": "sections", "workload": sc["name"], "steps": steps, "engines": {}}
+    for name, engine in (("fused", fsm.ENGINE_FUSED), ("reference", fsm.ENGINE_REFERENCE)):
+        t = fsm.FluidStep(gpu_params(sc, engine), sc["pos"], sc["vel"])
+        t.set_timing(True)
+        t.step(1)
+        t.step(steps)
+        coarse = t.stats()
+        t.set_detailed_timing(True)
+        t.step(steps)
+        d = t.detailed_timing()
+        out["engines"][name] = {
+            "coarse_ms": {k: coarse[k] for k in ("ms_prep", "ms_pressure", "ms_finish", "ms_total", "ms_cg")},
+            "detailed": d, "dispatches_per_substep": t.dispatches_per_substep(),
+            "stats_json": json.loads(t.stats_json()),
+        }
+        print(f"sections: {name} done", file=sys.stderr, flush=True)
+        t.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -456,6 +482,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="adjoint report: skip the FP64 reference gradient")
     ap.add_argument("--view-report", action="store_true",
                     help="instead of timing the step, time the live-buffer hand-off and the vec4 particle pack")
+    ap.add_argument("--sections-report", action="store_true",
+                    help="instead of timing the step, report both engines' per-section device time under the "
+                         "reference's section names")
     ap.add_argument("--scene-report", action="store_true",
                     help="instead of timing the step, time scene seeding on the GPU against the CPU restatement")
     ap.add_argument("--workload", default="high_density_128",
@@ -489,6 +518,10 @@ def main():
 
     if args.scene_report:
         print(json.dumps(scene_report(fsm)), flush=True)
+        return
+
+    if args.sections_report:
+        print(json.dumps(sections_report(fsm, scenes, gpu_params)), flush=True)
         return
 
     if args.view_report:

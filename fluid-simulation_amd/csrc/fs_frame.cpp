@@ -464,6 +464,12 @@ FS_API int fs_frame_reset(fs_frame* f) {
 }
 
 FS_API uint64_t fs_frame_tick(const fs_frame* f) { return f ? f->tick : 0u; }
+FS_API int fs_frame_timestep_stats(const fs_frame* f, fs_timestep_stats* out, float* total_dropped_s) {
+    if (!f || !out) return fail(FS_ERR_INVALID_ARGUMENT, "null argument");
+    *out = f->timestep.last;
+    if (total_dropped_s) *total_dropped_s = f->timestep.dropped_time;
+    return FS_OK;
+}
 FS_API int fs_frame_orientation(const fs_frame* f, float q_out[4]) {
     if (!f || !q_out) return fail(FS_ERR_INVALID_ARGUMENT, "null argument");
     q_out[0] = f->box_orient.x;

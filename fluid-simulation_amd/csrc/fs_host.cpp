@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,7 @@ using namespace fs;
 // Error reporting
 namespace fs {
 thread_local std::string g_last_error;
+thread_local uint64_t g_dispatches = 0;
 
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
@@ -271,6 +273,35 @@ int upload_particles(fs_tank* t, const float* pos, const float* vel) {
     return FS_OK;
 }
 
+// Detailed timing: records a boundary event closing the span since the
+// previous one under `tag` (a section index, kTagCg + bucket, kTagCgSolve).
+constexpr int kTagCg = 100, kTagCgSolve = 200, kTagStart = -1;
+// FINE_SECTIONS indices (src/gpu/timing.rs:32-59).
+enum : int {
+    kSecClear, kSecMark, kSecSort, kSecClassify, kSecScatter, kSecNormalize, kSecSaveVel = 8, kSecForces = 11,
+    kSecBoundPre = 14, kSecDivergence = 17, kSecCgInit, kSecGradient, kSecBoundPost = 22, kSecG2p = 25
+};
+constexpr int kFineSections = 26;
+// CG_CATS indices (src/gpu/timing.rs:105).
+enum : int { kCatSpmv, kCatReduce, kCatUpdate, kCatScalars, kCgCategories };
+
+int mark(fs_tank* t, int tag) {
+    if (!t->dev_armed) return FS_OK;
+    if (t->dev_n == t->dev_ev.size()) {
+        hipEvent_t e = nullptr;
+        FS_HIP(hipEventCreate(&e));
+        t->dev_ev.push_back(e);
+        t->dev_tag.push_back(0);
+    }
+    FS_HIP(hipEventRecord(t->dev_ev[t->dev_n], t->stream));
+    t->dev_tag[t->dev_n++] = tag;
+    return FS_OK;
+}
+#define FS_MARK(t, tag)                         \
+    do {                                        \
+        if (int rc_ = mark((t), (tag))) return rc_; \
+    } while (0)
+
 // Stable sort of the live side into the other side (both engines).
 int sort_particles(fs_tank* t) {
     const StepParams& P = t->P;
@@ -289,6 +320,7 @@ int substep_reference(fs_tank* t, bool sort_now) {
     const StepParams& P = t->P;
     hipStream_t s = t->stream;
     if (t->timing) FS_HIP(hipEventRecord(t->ev[0], s));
+    FS_MARK(t, kTagStart);
     for (int a = 0; a < 3; ++a) {
         FS_HIP(launch_fill_u32((uint32_t*)t->num[a], 0u, P.F[a], s));
         FS_HIP(launch_fill_u32((uint32_t*)t->den[a], 0u, P.F[a], s));
@@ -296,7 +328,9 @@ int substep_reference(fs_tank* t, bool sort_now) {
     FS_HIP(launch_fill_u32(t->occ, 0u, P.C, s));
     FS_HIP(launch_fill_u32(t->counters, 0u, 4, s));
     if (!P.warm) FS_HIP(launch_fill_f32(t->p, 0.0f, P.C, s));
+    FS_MARK(t, kSecClear);
     FS_HIP(launch_mark(P, t->side[t->cur].pos, t->occ, s));
+    FS_MARK(t, kSecMark);
     if (sort_now) {
         // Keys for the sort: same expression as mark; the histogram it also
         // produces goes to scratch (cell_offset is rewritten by the scan).
@@ -304,36 +338,68 @@ int substep_reference(fs_tank* t, bool sort_now) {
         int rc = sort_particles(t);
         if (rc != FS_OK) return rc;
     }
+    FS_MARK(t, kSecSort);  // an empty span on substeps without a sort
     ParticleSoA live = t->side[t->cur];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
+    FS_MARK(t, kSecClassify);
     FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, 0u, s));
-    for (int a = 0; a < 3; ++a) FS_HIP(launch_normalize(P, a, t->num[a], t->den[a], t->vel[a], s));
-    for (int a = 0; a < 3; ++a) FS_HIP(launch_copy_f32(t->vel[a], t->saved[a], P.F[a], s));
-    for (int a = 0; a < 3; ++a) FS_HIP(launch_forces(P, a, t->ct, t->vel[a], s));
-    for (int a = 0; a < 3; ++a) FS_HIP(launch_enforce(P, a, t->ct, t->vel[a], s));
+    FS_MARK(t, kSecScatter);
+    for (int a = 0; a < 3; ++a) {
+        FS_HIP(launch_normalize(P, a, t->num[a], t->den[a], t->vel[a], s));
+        FS_MARK(t, kSecNormalize + a);
+    }
+    for (int a = 0; a < 3; ++a) {
+        FS_HIP(launch_copy_f32(t->vel[a], t->saved[a], P.F[a], s));
+        FS_MARK(t, kSecSaveVel + a);
+    }
+    for (int a = 0; a < 3; ++a) {
+        FS_HIP(launch_forces(P, a, t->ct, t->vel[a], s));
+        FS_MARK(t, kSecForces + a);
+    }
+    for (int a = 0; a < 3; ++a) {
+        FS_HIP(launch_enforce(P, a, t->ct, t->vel[a], s));
+        FS_MARK(t, kSecBoundPre + a);
+    }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[1], s));
     FS_HIP(launch_divergence(P, t->vel, t->ct, t->occ, t->div, s));
+    FS_MARK(t, kSecDivergence);
     FS_HIP(launch_cg_init_ref(P, t->div, t->ct, t->p, t->r, t->d0, t->scal, s));
     FS_HIP(launch_cg_reduce_ref(P, t->r, t->r, t->partials, t->scal, s));
     FS_HIP(launch_cg_reduce_final_ref(P, t->partials, t->scal, s));
     FS_HIP(launch_cg_set_rsold_ref(t->scal, s));
+    FS_MARK(t, kSecCgInit);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[4], s));
+    // Six timed passes per iteration in solver order, bucketed as the
+    // reference's CG_BUCKET (src/gpu/timing.rs:110-116).
     for (uint32_t it = 0; it < P.iters; ++it) {
         FS_HIP(launch_cg_iter_count_ref(t->scal, t->counters + 2, s));
         FS_HIP(launch_cg_spmv_ref(P, t->ct, t->d0, t->q, t->scal, s));
+        FS_MARK(t, kTagCg + kCatSpmv);
         FS_HIP(launch_cg_reduce_ref(P, t->d0, t->q, t->partials, t->scal, s));
         FS_HIP(launch_cg_reduce_final_ref(P, t->partials, t->scal, s));
+        FS_MARK(t, kTagCg + kCatReduce);
         FS_HIP(launch_cg_alpha_ref(t->scal, s));
+        FS_MARK(t, kTagCg + kCatScalars);
         FS_HIP(launch_cg_update_ref(P, t->scal, t->d0, t->q, t->p, t->r, s));
+        FS_MARK(t, kTagCg + kCatUpdate);
         FS_HIP(launch_cg_reduce_ref(P, t->r, t->r, t->partials, t->scal, s));
         FS_HIP(launch_cg_reduce_final_ref(P, t->partials, t->scal, s));
+        FS_MARK(t, kTagCg + kCatReduce);
         FS_HIP(launch_cg_beta_ref(t->scal, s));
         FS_HIP(launch_cg_dir_ref(P, t->scal, t->r, t->d0, s));
+        FS_MARK(t, kTagCg + kCatScalars);
     }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[2], s));
-    for (int a = 0; a < 3; ++a) FS_HIP(launch_gradient(P, a, t->p, t->ct, t->vel[a], s));
-    for (int a = 0; a < 3; ++a) FS_HIP(launch_enforce(P, a, t->ct, t->vel[a], s));
+    for (int a = 0; a < 3; ++a) {
+        FS_HIP(launch_gradient(P, a, t->p, t->ct, t->vel[a], s));
+        FS_MARK(t, kSecGradient + a);
+    }
+    for (int a = 0; a < 3; ++a) {
+        FS_HIP(launch_enforce(P, a, t->ct, t->vel[a], s));
+        FS_MARK(t, kSecBoundPost + a);
+    }
     FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
+    FS_MARK(t, kSecG2p);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[3], s));
     t->gather_last = 0;
     return FS_OK;
@@ -348,15 +414,22 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     const StepParams& P = t->P;
     hipStream_t s = t->stream;
     if (t->timing) FS_HIP(hipEventRecord(t->ev[0], s));
+    // Detailed timing: each fused kernel is timed under the first section
+    // whose work it does (the rest report as fused, zero span).
+    FS_MARK(t, kTagStart);
     FS_HIP(launch_fill_u32(t->occ, 0u, P.C, s));
     FS_HIP(launch_fill_u32(t->counters, 0u, 4, s));
+    FS_MARK(t, kSecClear);
     FS_HIP(launch_key_hist(P, t->side[t->cur].pos, t->keys, t->occ, t->counters + 1, t->mis_list, kMisfitCap, s));
+    FS_MARK(t, kSecMark);
     if (sort_now) {
         int rc = sort_particles(t);
         if (rc != FS_OK) return rc;
     }
+    FS_MARK(t, kSecSort);
     ParticleSoA live = t->side[t->cur];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
+    FS_MARK(t, kSecClassify);
     if (sort_now) {
         FS_HIP(launch_p2g_gather(P, live, t->side[t->cur ^ 1], t->bf16, t->occ, t->cell_offset, t->ct, t->vel,
                                  t->saved, t->counters + 1, t->mis_list, s, t->gather_brick));
@@ -367,7 +440,9 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
             FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
         }
         FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
+        FS_MARK(t, kSecScatter);
         FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
+        FS_MARK(t, kSecNormalize);
         t->gather_last = 1;
     } else {
         for (int a = 0; a < 3; ++a) {
@@ -375,12 +450,15 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
             FS_HIP(launch_fill_u32((uint32_t*)t->den[a], 0u, P.F[a], s));
         }
         FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, 0u, s));
+        FS_MARK(t, kSecScatter);
         FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, nullptr, 0u, s));
+        FS_MARK(t, kSecNormalize);
         t->gather_last = 0;
     }
     if (t->timing) FS_HIP(hipEventRecord(t->ev[1], s));
     FS_HIP(launch_div_cginit(P, t->vel, t->ct, t->occ, t->div, t->b, t->p, t->r, t->d0, t->cg, t->scan_ws, s));
     FS_HIP(launch_cg_reset(t->cg, s));
+    FS_MARK(t, kSecDivergence);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[4], s));
     if (t->cg_mode == 0) {
         FS_HIP(launch_cg_persistent(P, t->cg_vectors(), t->cg, t->cg_blocks, s));
@@ -392,8 +470,11 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
             // dropped whenever the step constants change.
             hipGraph_t g = nullptr;
             FS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const uint64_t before = g_dispatches;
             hipError_t le =
                 launch_cg_multi(P, t->ct, t->b, t->p, t->r, t->d0, t->d1, t->q, t->cg, t->cg_multi_blocks, s);
+            t->cg_graph_dispatches = g_dispatches - before;  // issued by each replay, not by the capture
+            g_dispatches = before;
             hipError_t ce = hipStreamEndCapture(s, &g);
             if (le != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -405,17 +486,47 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
             FS_HIP(ie);
         }
         FS_HIP(hipGraphLaunch(t->cg_exec, s));
+        g_dispatches += t->cg_graph_dispatches;
     }
+    FS_MARK(t, kTagCgSolve);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[2], s));
     FS_HIP(launch_gradient_enforce(P, t->p, t->ct, t->vel, s));
+    FS_MARK(t, kSecGradient);
     if (stop_before_g2p) return FS_OK;
     FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
+    FS_MARK(t, kSecG2p);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[3], s));
+    return FS_OK;
+}
+
+// Adds the spans of the substep's boundary events to the detailed totals.
+int collect_detailed(fs_tank* t) {
+    if (t->dev_n == 0) return FS_OK;
+    FS_HIP(hipEventSynchronize(t->dev_ev[t->dev_n - 1]));
+    for (size_t i = 1; i < t->dev_n; ++i) {
+        float ms = 0.0f;
+        FS_HIP(hipEventElapsedTime(&ms, t->dev_ev[i - 1], t->dev_ev[i]));
+        const int tag = t->dev_tag[i];
+        if (tag >= 0 && tag < kFineSections) t->sec_ms[tag] += ms;
+        else if (tag >= kTagCg && tag < kTagCg + kCgCategories) t->cg_cat_ms[tag - kTagCg] += ms;
+        if (tag >= kTagCg) t->cg_total_ms += ms;
+    }
+    t->dev_n = 0;
+    t->substeps_timed += 1;
     return FS_OK;
 }
 
 int do_step(fs_tank* t, uint32_t substeps) {
     double acc[4] = {0, 0, 0, 0};
+    if (t->detailed) {
+        // Totals over this call's substeps (GpuTimers' frame totals).
+        std::fill(std::begin(t->sec_ms), std::end(t->sec_ms), 0.0);
+        std::fill(std::begin(t->cg_cat_ms), std::end(t->cg_cat_ms), 0.0);
+        t->cg_total_ms = 0.0;
+        t->substeps_timed = 0;
+        t->cg_iters_timed = t->P.iters;
+    }
+    t->call_substeps = substeps;
     for (uint32_t i = 0; i < substeps; ++i) {
         bool sort_now = false;
         if (t->prm.particle_sort) {
@@ -423,10 +534,17 @@ int do_step(fs_tank* t, uint32_t substeps) {
             sort_now = (t->tick % period) == 0;
             t->tick += 1;
         }
+        const uint64_t d0 = g_dispatches;
+        t->dev_armed = t->detailed != 0;
+        t->dev_n = 0;
         int rc = t->prm.engine == FS_ENGINE_FUSED ? substep_fused(t, sort_now) : substep_reference(t, sort_now);
+        t->dev_armed = false;
         if (rc != FS_OK) return rc;
+        t->dispatches_last = (uint32_t)(g_dispatches - d0);
         t->sorted_last = sort_now ? 1 : 0;
         t->substeps += 1;
+        if (t->detailed)
+            if (int rc2 = collect_detailed(t)) return rc2;
         if (t->timing) {
             FS_HIP(hipEventSynchronize(t->ev[3]));
             float a = 0, b = 0, c = 0, g = 0;

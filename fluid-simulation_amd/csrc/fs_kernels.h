@@ -13,6 +13,15 @@ struct ParticleSoA {
     void* vel[3];
 };
 
+// Kernel dispatches issued by this thread's launchers (each launcher counts
+// what it enqueues); the host reads the difference across a substep.
+extern thread_local uint64_t g_dispatches;
+#define FS_LAUNCH(...)               \
+    do {                             \
+        ++::fs::g_dispatches;        \
+        hipLaunchKernelGGL(__VA_ARGS__); \
+    } while (0)
+
 // Listed misfit particles (P2G cell != sort cell) the brick gather walks;
 // beyond this many the step takes the atomic P2G form instead.
 constexpr uint32_t kMisfitCap = 4096;

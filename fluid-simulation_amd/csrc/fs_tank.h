@@ -58,6 +58,21 @@ struct fs_tank {
     int sorted_last = 0, gather_last = 0, perm_valid = 0;
     int timing = 0;
     hipEvent_t ev[5] = {};
+    // Detailed timing (the reference's FINE_SECTIONS and CG_CATS): boundary
+    // events of the substep in flight, each closing a span with a tag
+    // (0..25 a section, 100 + c a CG category, 200 the whole fused solve).
+    int detailed = 0;
+    bool dev_armed = false;
+    std::vector<hipEvent_t> dev_ev;
+    std::vector<int> dev_tag;
+    size_t dev_n = 0;
+    double sec_ms[26] = {}, cg_cat_ms[4] = {}, cg_total_ms = 0.0;
+    uint32_t cg_iters_timed = 0, substeps_timed = 0;
+    // Kernel dispatches of the last substep, and those one replay of the
+    // captured CG graph issues.
+    uint32_t dispatches_last = 0;
+    uint32_t call_substeps = 0;  // substeps of the last fs_step call
+    uint64_t cg_graph_dispatches = 0;
     double ms[5] = {0, 0, 0, 0, 0};
     uint64_t bytes = 0;
 
@@ -123,6 +138,7 @@ struct fs_tank {
         for (void* a : allocs) (void)hipFree(a);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : dev_ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

@@ -53,7 +53,7 @@ hipError_t launch_fill_u32_guarded(uint32_t* p, uint32_t value, size_t count, co
     if (count == 0) return hipSuccess;
     unsigned g = grid_for(count, kBlock);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_fill_u32_guarded, dim3(g), dim3(kBlock), 0, s, p, value, count, guard, above);
+    FS_LAUNCH(k_fill_u32_guarded, dim3(g), dim3(kBlock), 0, s, p, value, count, guard, above);
     return hipGetLastError();
 }
 
@@ -61,7 +61,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
     if (count == 0) return hipSuccess;
     unsigned g = grid_for(count, kBlock);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    FS_LAUNCH(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, value, count);
     return hipGetLastError();
 }
 
@@ -69,7 +69,7 @@ hipError_t launch_fill_f32(float* p, float value, size_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     unsigned g = grid_for(count, kBlock);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_fill_f32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    FS_LAUNCH(k_fill_f32, dim3(g), dim3(kBlock), 0, s, p, value, count);
     return hipGetLastError();
 }
 
@@ -83,7 +83,7 @@ __global__ void k_mark(StepParams P, const float* __restrict__ px, const float* 
 
 hipError_t launch_mark(const StepParams& P, const float* const pos[3], uint32_t* occ, hipStream_t s) {
     if (P.N == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mark, dim3(grid_for(P.N, 64)), dim3(64), 0, s, P, pos[0], pos[1], pos[2], occ);
+    FS_LAUNCH(k_mark, dim3(grid_for(P.N, 64)), dim3(64), 0, s, P, pos[0], pos[1], pos[2], occ);
     return hipGetLastError();
 }
 
@@ -112,7 +112,7 @@ __global__ void k_key_hist(StepParams P, const float* __restrict__ px, const flo
 hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint32_t* keys, uint32_t* occ,
                            uint32_t* mis_count, uint32_t* mis_list, uint32_t mis_cap, hipStream_t s) {
     if (P.N == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_key_hist, dim3(grid_for(P.N, kBlock)), dim3(kBlock), 0, s, P, pos[0], pos[1],
+    FS_LAUNCH(k_key_hist, dim3(grid_for(P.N, kBlock)), dim3(kBlock), 0, s, P, pos[0], pos[1],
                        pos[2], keys, occ, mis_count, mis_list, mis_cap);
     return hipGetLastError();
 }
@@ -198,15 +198,15 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, 
     if (len == 0) return hipSuccess;
     size_t tiles = (len + kScanTile - 1) / kScanTile;
     if (tiles == 1) {
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, in, out, (uint32_t*)nullptr, len);
+        FS_LAUNCH(k_scan_tiles, dim3(1), dim3(kBlock), 0, s, in, out, (uint32_t*)nullptr, len);
         return hipGetLastError();
     }
     uint32_t* sums = ws;
     uint32_t* sums_scanned = ws + tiles;
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, out, sums, len);
+    FS_LAUNCH(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, out, sums, len);
     hipError_t e = launch_exclusive_scan(sums, sums_scanned, tiles, ws + 2 * tiles, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kBlock), 0, s, out, sums_scanned, len);
+    FS_LAUNCH(k_scan_add, dim3((unsigned)tiles), dim3(kBlock), 0, s, out, sums_scanned, len);
     return hipGetLastError();
 }
 
@@ -367,24 +367,24 @@ hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t 
     for (uint32_t ps = 0; ps < passes; ++ps) {
         const uint32_t shift = 8 * ps;
         if (items == 16)
-            hipLaunchKernelGGL(k_radix_hist<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+            FS_LAUNCH(k_radix_hist<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         else if (items == 8)
-            hipLaunchKernelGGL(k_radix_hist<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+            FS_LAUNCH(k_radix_hist<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         else
-            hipLaunchKernelGGL(k_radix_hist<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+            FS_LAUNCH(k_radix_hist<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         hipError_t e = launch_exclusive_scan(hist, offs, table, scan_ws, s);
         if (e != hipSuccess) return e;
         uint32_t* kout = kbuf[ps & 1];
         uint32_t* vout = vbuf[ps & 1];
         const int write_keys = ps + 1 < passes;
         if (items == 16)
-            hipLaunchKernelGGL(k_radix_scatter<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+            FS_LAUNCH(k_radix_scatter<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
                                n, shift, ntiles, write_keys);
         else if (items == 8)
-            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+            FS_LAUNCH(k_radix_scatter<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
                                n, shift, ntiles, write_keys);
         else
-            hipLaunchKernelGGL(k_radix_scatter<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+            FS_LAUNCH(k_radix_scatter<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
                                n, shift, ntiles, write_keys);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -411,8 +411,8 @@ __global__ void k_permute(uint32_t n, const uint32_t* __restrict__ perm, Particl
 hipError_t launch_permute(uint32_t n, const uint32_t* perm, const ParticleSoA& src, const ParticleSoA& dst,
                           int bf16, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (bf16) hipLaunchKernelGGL(k_permute<1>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, perm, src, dst);
-    else hipLaunchKernelGGL(k_permute<0>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, perm, src, dst);
+    if (bf16) FS_LAUNCH(k_permute<1>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, perm, src, dst);
+    else FS_LAUNCH(k_permute<0>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, perm, src, dst);
     return hipGetLastError();
 }
 

@@ -163,46 +163,46 @@ __global__ void k_cg_iter_count_ref(const float* scal, uint32_t* count) {
 
 hipError_t launch_cg_init_ref(const StepParams& P, const float* div, const uint8_t* ct, float* p, float* r,
                               float* d, float* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_init_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, div, ct, p, r, d, scal);
+    FS_LAUNCH(k_cg_init_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, div, ct, p, r, d, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_reduce_ref(const StepParams& P, const float* a, const float* b, float* partials,
                                 const float* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_reduce_ref, dim3(grid_for(P.C, 256)), dim3(256), 0, s, P, a, b, partials, scal);
+    FS_LAUNCH(k_cg_reduce_ref, dim3(grid_for(P.C, 256)), dim3(256), 0, s, P, a, b, partials, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_reduce_final_ref(const StepParams& P, const float* partials, float* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_reduce_final_ref, dim3(1), dim3(256), 0, s, P, partials, scal);
+    FS_LAUNCH(k_cg_reduce_final_ref, dim3(1), dim3(256), 0, s, P, partials, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_set_rsold_ref(float* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_set_rsold_ref, dim3(1), dim3(1), 0, s, scal);
+    FS_LAUNCH(k_cg_set_rsold_ref, dim3(1), dim3(1), 0, s, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_spmv_ref(const StepParams& P, const uint8_t* ct, const float* d, float* q, const float* scal,
                               hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_spmv_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, ct, d, q, scal);
+    FS_LAUNCH(k_cg_spmv_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, ct, d, q, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_alpha_ref(float* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha_ref, dim3(1), dim3(1), 0, s, scal);
+    FS_LAUNCH(k_cg_alpha_ref, dim3(1), dim3(1), 0, s, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_update_ref(const StepParams& P, const float* scal, const float* d, const float* q, float* p,
                                 float* r, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_update_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, scal, d, q, p, r);
+    FS_LAUNCH(k_cg_update_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, scal, d, q, p, r);
     return hipGetLastError();
 }
 hipError_t launch_cg_beta_ref(float* scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_beta_ref, dim3(1), dim3(1), 0, s, scal);
+    FS_LAUNCH(k_cg_beta_ref, dim3(1), dim3(1), 0, s, scal);
     return hipGetLastError();
 }
 hipError_t launch_cg_dir_ref(const StepParams& P, const float* scal, const float* r, float* d, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_dir_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, scal, r, d);
+    FS_LAUNCH(k_cg_dir_ref, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, scal, r, d);
     return hipGetLastError();
 }
 hipError_t launch_cg_iter_count_ref(const float* scal, uint32_t* count, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_iter_count_ref, dim3(1), dim3(1), 0, s, scal, count);
+    FS_LAUNCH(k_cg_iter_count_ref, dim3(1), dim3(1), 0, s, scal, count);
     return hipGetLastError();
 }
 

@@ -83,7 +83,7 @@ __global__ void k_compact_tiles(uint32_t ntiles, const uint32_t* __restrict__ fl
 
 hipError_t launch_compact_tiles(uint32_t ntiles, const uint32_t* flags, const uint32_t* scan, uint32_t* tiles,
                                 uint32_t* n_active, hipStream_t s) {
-    hipLaunchKernelGGL(k_compact_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, ntiles, flags, scan, tiles,
+    FS_LAUNCH(k_compact_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, ntiles, flags, scan, tiles,
                        n_active);
     return hipGetLastError();
 }
@@ -92,14 +92,14 @@ hipError_t launch_div_cginit(const StepParams& P, float* const vel[3], const uin
                              float* div, float* b, float* p, float* r, float* d, const CgWork& w, uint32_t* scan_ws,
                              hipStream_t s) {
     // blockDim is a multiple of 64, so each wave is exactly one tile.
-    hipLaunchKernelGGL(k_div_cginit, dim3(grid_for(P.C, 256)), dim3(256), 0, s, P, vel[0], vel[1], vel[2], ct, occ,
+    FS_LAUNCH(k_div_cginit, dim3(grid_for(P.C, 256)), dim3(256), 0, s, P, vel[0], vel[1], vel[2], ct, occ,
                        div, b, p, r, d, w.nmask, w.tile_flags);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t ntiles = (P.C + kTile - 1) / kTile;
     e = launch_exclusive_scan(w.tile_flags, w.tile_scan, ntiles, scan_ws, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_compact_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, ntiles, w.tile_flags,
+    FS_LAUNCH(k_compact_tiles, dim3(grid_for(ntiles, 256)), dim3(256), 0, s, ntiles, w.tile_flags,
                        w.tile_scan, w.tiles, w.n_tiles);
     return hipGetLastError();
 }
@@ -736,6 +736,7 @@ hipError_t launch_cg_persistent(const StepParams& P, const CgVectors& v, const C
     const PersistentVariant pv = persistent_variant();
     if (pv.single) {
         void* args[] = {(void*)&P, (void*)&v, (void*)&w};
+        ++g_dispatches;
         return hipLaunchCooperativeKernel(pv.fn, dim3(grid_blocks), dim3(pv.threads), args, 0, s);
     }
     const uint8_t* ct = nullptr;
@@ -743,6 +744,7 @@ hipError_t launch_cg_persistent(const StepParams& P, const CgVectors& v, const C
     float *p = v.x, *r = v.r[0], *d0 = v.pd, *d1 = v.s[0], *q = v.w[0];
     void* args[] = {(void*)&P, (void*)&ct, (void*)&b, (void*)&p, (void*)&r,
                     (void*)&d0, (void*)&d1, (void*)&q, (void*)&w};
+    ++g_dispatches;
     return hipLaunchCooperativeKernel(pv.fn, dim3(grid_blocks), dim3(pv.threads), args, 0, s);
 }
 
@@ -831,7 +833,7 @@ hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* 
                            float* d1, float* q, const CgWork& w, int grid_blocks, hipStream_t s) {
     (void)ct;
     const dim3 g(grid_blocks), t(kCgMultiThreads);
-    hipLaunchKernelGGL(k_cg_rs0, g, t, 0, s, P, r, w);
+    FS_LAUNCH(k_cg_rs0, g, t, 0, s, P, r, w);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     float* D[2] = {d0, d1};
@@ -840,10 +842,10 @@ hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* 
         const float* dprev = first ? d0 : D[(it + 1u) & 1u];
         float* dnext = first ? d1 : D[it & 1u];
         const float* dn = first ? d0 : D[it & 1u];
-        hipLaunchKernelGGL(k_cg_phase_a, g, t, 0, s, P, r, dprev, dnext, q, first, w);
-        hipLaunchKernelGGL(k_cg_phase_b, g, t, 0, s, P, dn, q, p, r, w);
+        FS_LAUNCH(k_cg_phase_a, g, t, 0, s, P, r, dprev, dnext, q, first, w);
+        FS_LAUNCH(k_cg_phase_b, g, t, 0, s, P, dn, q, p, r, w);
     }
-    hipLaunchKernelGGL(k_cg_true_res, g, t, 0, s, P, b, p, w);
+    FS_LAUNCH(k_cg_true_res, g, t, 0, s, P, b, p, w);
     return hipGetLastError();
 }
 

@@ -30,10 +30,10 @@ hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16, co
     if (P.N == 0) return hipSuccess;
     const unsigned G = (grid_for(P.N, 256) + 7u) / 8u * 8u;  // whole XCD groups for xcd_remap
     if (bf16)
-        hipLaunchKernelGGL(k_g2p<1>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
+        FS_LAUNCH(k_g2p<1>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
                            saved[2]);
     else
-        hipLaunchKernelGGL(k_g2p<0>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
+        FS_LAUNCH(k_g2p<0>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
                            saved[2]);
     return hipGetLastError();
 }
@@ -50,8 +50,8 @@ __global__ void k_impulse(uint32_t n, ParticleSoA sp, float dx, float dy, float 
 hipError_t launch_impulse(const StepParams& P, const ParticleSoA& live, int bf16, float dx, float dy, float dz,
                           hipStream_t s) {
     if (P.N == 0) return hipSuccess;
-    if (bf16) hipLaunchKernelGGL(k_impulse<1>, dim3(grid_for(P.N, 256)), dim3(256), 0, s, P.N, live, dx, dy, dz);
-    else hipLaunchKernelGGL(k_impulse<0>, dim3(grid_for(P.N, 256)), dim3(256), 0, s, P.N, live, dx, dy, dz);
+    if (bf16) FS_LAUNCH(k_impulse<1>, dim3(grid_for(P.N, 256)), dim3(256), 0, s, P.N, live, dx, dy, dz);
+    else FS_LAUNCH(k_impulse<0>, dim3(grid_for(P.N, 256)), dim3(256), 0, s, P.N, live, dx, dy, dz);
     return hipGetLastError();
 }
 

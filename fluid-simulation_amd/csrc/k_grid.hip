@@ -40,7 +40,7 @@ __global__ void k_classify(StepParams P, const uint32_t* __restrict__ occ, uint8
 
 hipError_t launch_classify(const StepParams& P, const uint32_t* occ, uint8_t* ct, uint32_t* stats,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_classify, dim3(grid_for(P.C, 1024)), dim3(1024), 0, s, P, occ, ct, stats);
+    FS_LAUNCH(k_classify, dim3(grid_for(P.C, 1024)), dim3(1024), 0, s, P, occ, ct, stats);
     return hipGetLastError();
 }
 
@@ -66,10 +66,10 @@ hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf
     unsigned g = grid_for(P.N, 64);
     if (guard && g > 4096) g = 4096;
     if (bf16)
-        hipLaunchKernelGGL(k_p2g_atomic<1>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2],
+        FS_LAUNCH(k_p2g_atomic<1>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2],
                            den[2], guard, above);
     else
-        hipLaunchKernelGGL(k_p2g_atomic<0>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2],
+        FS_LAUNCH(k_p2g_atomic<0>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2],
                            den[2], guard, above);
     return hipGetLastError();
 }
@@ -118,25 +118,25 @@ __global__ void k_gradient(StepParams P, int a, const float* __restrict__ pr, co
 
 hipError_t launch_normalize(const StepParams& P, int a, const int32_t* num, const int32_t* den, float* vel,
                             hipStream_t s) {
-    hipLaunchKernelGGL(k_normalize, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P.F[a], num, den, vel);
+    FS_LAUNCH(k_normalize, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P.F[a], num, den, vel);
     return hipGetLastError();
 }
 hipError_t launch_copy_f32(const float* src, float* dst, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 64)), dim3(64), 0, s, src, dst, n);
+    FS_LAUNCH(k_copy, dim3(grid_for(n, 64)), dim3(64), 0, s, src, dst, n);
     return hipGetLastError();
 }
 hipError_t launch_forces(const StepParams& P, int a, const uint8_t* ct, float* vel, hipStream_t s) {
-    hipLaunchKernelGGL(k_forces, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P, a, ct, vel);
+    FS_LAUNCH(k_forces, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P, a, ct, vel);
     return hipGetLastError();
 }
 hipError_t launch_enforce(const StepParams& P, int a, const uint8_t* ct, float* vel, hipStream_t s) {
-    hipLaunchKernelGGL(k_enforce, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P, a, ct, vel);
+    FS_LAUNCH(k_enforce, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P, a, ct, vel);
     return hipGetLastError();
 }
 hipError_t launch_gradient(const StepParams& P, int a, const float* p, const uint8_t* ct, float* vel,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_gradient, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P, a, p, ct, vel);
+    FS_LAUNCH(k_gradient, dim3(grid_for(P.F[a], 64)), dim3(64), 0, s, P, a, p, ct, vel);
     return hipGetLastError();
 }
 
@@ -150,7 +150,7 @@ __global__ void k_divergence(StepParams P, const float* __restrict__ u, const fl
 
 hipError_t launch_divergence(const StepParams& P, float* const vel[3], const uint8_t* ct, const uint32_t* occ,
                              float* div, hipStream_t s) {
-    hipLaunchKernelGGL(k_divergence, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, vel[0], vel[1], vel[2], ct,
+    FS_LAUNCH(k_divergence, dim3(grid_for(P.C, 64)), dim3(64), 0, s, P, vel[0], vel[1], vel[2], ct,
                        occ, div);
     return hipGetLastError();
 }
@@ -189,7 +189,7 @@ hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3]
     const size_t tot = (size_t)P.F[0] + P.F[1] + P.F[2];
     unsigned g = grid_for(tot, 256);
     if (guard && g > 4096) g = 4096;  // usually a no-op fallback: small grid-stride grid
-    hipLaunchKernelGGL(k_face_epilogue, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
+    FS_LAUNCH(k_face_epilogue, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
                        ct, vel[0], vel[1], vel[2], saved[0], saved[1], saved[2], guard, above);
     return hipGetLastError();
 }
@@ -211,7 +211,7 @@ __global__ void k_gradient_enforce(StepParams P, const float* __restrict__ pr, c
 hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const uint8_t* ct, float* const vel[3],
                                    hipStream_t s) {
     const size_t tot = (size_t)P.F[0] + P.F[1] + P.F[2];
-    hipLaunchKernelGGL(k_gradient_enforce, dim3(grid_for(tot, 256)), dim3(256), 0, s, P, p, ct, vel[0], vel[1],
+    FS_LAUNCH(k_gradient_enforce, dim3(grid_for(tot, 256)), dim3(256), 0, s, P, p, ct, vel[0], vel[1],
                        vel[2]);
     return hipGetLastError();
 }
@@ -247,16 +247,16 @@ hipError_t launch_gather_v(const StepParams& P, const ParticleSoA& sorted, const
     const uint32_t G = nbx * nby * nbz;
     if (G <= kGSmallGrid) {
         if (bf16)
-            hipLaunchKernelGGL((k_p2g_gather<1, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
+            FS_LAUNCH((k_p2g_gather<1, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
                                occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
         else
-            hipLaunchKernelGGL((k_p2g_gather<0, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
+            FS_LAUNCH((k_p2g_gather<0, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
                                occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
     } else if (bf16) {
-        hipLaunchKernelGGL((k_p2g_gather<1, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
+        FS_LAUNCH((k_p2g_gather<1, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
                            occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
     } else {
-        hipLaunchKernelGGL((k_p2g_gather<0, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
+        FS_LAUNCH((k_p2g_gather<0, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
                            occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
     }
     return hipGetLastError();

@@ -50,8 +50,11 @@ EXPORTED = (
     "fs_timestep_total_dropped", "fs_interactions_create", "fs_interactions_destroy", "fs_interactions_reset",
     "fs_interactions_auto_roll", "fs_interactions_wave", "fs_interactions_next_u32", "fs_frame_create",
     "fs_frame_destroy", "fs_frame_run", "fs_frame_export", "fs_frame_set_paused", "fs_frame_queue_step",
-    "fs_frame_reset", "fs_frame_tick", "fs_frame_orientation",
+    "fs_frame_reset", "fs_frame_tick", "fs_frame_orientation", "fs_frame_timestep_stats",
+    "fs_set_detailed_timing", "fs_get_detailed_timing", "fs_fine_section_name", "fs_cg_category_name",
+    "fs_dispatches_per_substep", "fs_stats_json",
 )
+FINE_SECTIONS_COUNT, CG_CATEGORIES_COUNT = 26, 4
 
 
 class FluidStepError(RuntimeError):
@@ -144,6 +147,29 @@ class TimestepStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DetailedTiming(C.Structure):
+    """fs_detailed_timing: GpuTimers' detailed readout (src/gpu/timing.rs:125-140)."""
+    _fields_ = [
+        ("detailed", C.c_uint32), ("substeps", C.c_uint32), ("cg_iters", C.c_uint32), ("cg_split", C.c_uint32),
+        ("section_ms", C.c_double * FINE_SECTIONS_COUNT), ("cg_ms", C.c_double * CG_CATEGORIES_COUNT),
+        ("cg_total_ms", C.c_double), ("prep_ms", C.c_double), ("pressure_ms", C.c_double),
+        ("finish_ms", C.c_double), ("fused", C.c_uint32 * FINE_SECTIONS_COUNT),
+    ]
+
+    def as_dict(self):
+        secs, cats = fine_sections(), cg_categories()
+        return {
+            "detailed": bool(self.detailed), "substeps": self.substeps, "cg_iters": self.cg_iters,
+            "cg_split": bool(self.cg_split),
+            "sections": {secs[i]: self.section_ms[i] for i in range(FINE_SECTIONS_COUNT)},
+            "fused": [secs[i] for i in range(FINE_SECTIONS_COUNT) if self.fused[i]],
+            "cg": {cats[c]: (None if math.isnan(self.cg_ms[c]) else self.cg_ms[c])
+                   for c in range(CG_CATEGORIES_COUNT)},
+            "cg_total_ms": self.cg_total_ms, "prep_ms": self.prep_ms, "pressure_ms": self.pressure_ms,
+            "finish_ms": self.finish_ms,
+        }
 
 
 class DeviceView(C.Structure):
@@ -281,6 +307,16 @@ def lib():
         L.fs_frame_tick.argtypes = [vp]
         L.fs_frame_tick.restype = C.c_uint64
         L.fs_frame_orientation.argtypes = [vp, C.POINTER(C.c_float)]
+        L.fs_frame_timestep_stats.argtypes = [vp, C.POINTER(TimestepStats), C.POINTER(C.c_float)]
+        L.fs_set_detailed_timing.argtypes = [vp, C.c_int]
+        L.fs_get_detailed_timing.argtypes = [vp, C.POINTER(DetailedTiming)]
+        L.fs_fine_section_name.argtypes = [C.c_int]
+        L.fs_fine_section_name.restype = C.c_char_p
+        L.fs_cg_category_name.argtypes = [C.c_int]
+        L.fs_cg_category_name.restype = C.c_char_p
+        L.fs_dispatches_per_substep.argtypes = [vp]
+        L.fs_dispatches_per_substep.restype = C.c_uint32
+        L.fs_stats_json.argtypes = [vp, vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.fs_get_device_view.argtypes = [vp, C.POINTER(DeviceView)]
         L.fs_take_sort_swapped.argtypes = [vp]
         L.fs_pack_particles_vec4.argtypes = [vp, vp]
@@ -468,6 +504,22 @@ class Frame:
         _check(lib().fs_frame_orientation(self._h, q))
         return np.array(list(q), np.float32)
 
+    def timestep_stats(self):
+        """(last frame's TimestepStats as a dict, total dropped simulation time in s)."""
+        st, dropped = TimestepStats(), C.c_float()
+        _check(lib().fs_frame_timestep_stats(self._h, C.byref(st), C.byref(dropped)))
+        return st.as_dict(), dropped.value
+
+
+def fine_sections() -> list:
+    """FINE_SECTIONS (src/gpu/timing.rs:32-59) as the library names them."""
+    return [lib().fs_fine_section_name(i).decode() for i in range(FINE_SECTIONS_COUNT)]
+
+
+def cg_categories() -> list:
+    """CG_CATS (src/gpu/timing.rs:105) as the library names them."""
+    return [lib().fs_cg_category_name(i).decode() for i in range(CG_CATEGORIES_COUNT)]
+
 
 def scene_preset_blocks(preset: int, drop_height: float, fill_level: float) -> np.ndarray:
     """preset_blocks (src/scene/mod.rs:208-238): (n, 6) float32, min xyz then max xyz."""
@@ -650,6 +702,27 @@ class FluidStep:
 
     def buffer_bytes(self) -> int:
         return int(lib().fs_buffer_bytes(self._h))
+
+    # timing and stats contract
+    def set_detailed_timing(self, on: bool):
+        _check(lib().fs_set_detailed_timing(self._h, int(on)))
+
+    def detailed_timing(self) -> dict:
+        d = DetailedTiming()
+        _check(lib().fs_get_detailed_timing(self._h, C.byref(d)))
+        return d.as_dict()
+
+    def dispatches_per_substep(self) -> int:
+        return int(lib().fs_dispatches_per_substep(self._h))
+
+    def stats_json(self, frame: "Frame" = None) -> str:
+        """Profiler::stats_json's simulation fields (src/profiler/mod.rs:416-594)."""
+        fh = frame._h if frame is not None else None
+        n = C.c_size_t()
+        _check(lib().fs_stats_json(self._h, fh, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _check(lib().fs_stats_json(self._h, fh, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
 
     # differentiation
     def forward_checkpointed(self, steps: int, every: int = 10):

@@ -16,6 +16,7 @@
 #ifndef FLUIDSTEP_H
 #define FLUIDSTEP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -362,6 +363,54 @@ FS_API int fs_frame_queue_step(fs_frame* f); /* the step button while paused */
 FS_API int fs_frame_reset(fs_frame* f);
 FS_API uint64_t fs_frame_tick(const fs_frame* f);
 FS_API int fs_frame_orientation(const fs_frame* f, float q_out[4]); /* box_orient */
+/* The timestep stats of the last frame and the total dropped simulation time
+ * (s), the pair FluidApp hands to Profiler::set_timestep_stats
+ * (src/profiler/mod.rs:172-193). */
+FS_API int fs_frame_timestep_stats(const fs_frame* f, fs_timestep_stats* out, float* total_dropped_s);
+
+/* ---- timing and stats contract ----
+ * GpuTimers' detailed mode (src/gpu/timing.rs:32-116, recorded by
+ * record_substep_detailed, src/gpu/mod.rs:1039-1212): device time per fine
+ * section (FINE_SECTIONS) and per CG category (CG_CATS, the six passes of an
+ * iteration bucketed as CG_BUCKET), summed over the substeps of the last
+ * fs_step call, from HIP events on the tank's stream.  Consecutive events
+ * share a boundary, so the sections tile the substep.  The reference engine
+ * times every section and the four categories.  The fused engine times each
+ * of its kernels under the first section whose work it does (the others are
+ * flagged fused and read 0) and the CG as one persistent solve
+ * (cg_split = 0, the categories NaN).  Detailed timing adds one event per
+ * section and six per CG iteration and waits for each substep. */
+#define FS_FINE_SECTIONS 26
+#define FS_CG_CATEGORIES 4
+typedef struct {
+    uint32_t detailed;    /* detailed timing was on for the last fs_step */
+    uint32_t substeps;    /* substeps summed (Readout.substeps) */
+    uint32_t cg_iters;    /* CG iterations per substep (Readout.cg_iters) */
+    uint32_t cg_split;    /* 1: cg_ms holds the categories */
+    double section_ms[FS_FINE_SECTIONS];
+    double cg_ms[FS_CG_CATEGORIES];
+    double cg_total_ms;
+    /* Readout.prep_ms / pressure_ms / finish_ms: the sections rolled up at
+     * PREP_END ("divergence") and FINISH_START ("gradient_u"), the CG in
+     * pressure (src/gpu/timing.rs:62-70, 472-481). */
+    double prep_ms, pressure_ms, finish_ms;
+    uint32_t fused[FS_FINE_SECTIONS]; /* 1: done inside an earlier section's kernel */
+} fs_detailed_timing;
+FS_API int fs_set_detailed_timing(fs_tank* t, int enabled);
+FS_API int fs_get_detailed_timing(fs_tank* t, fs_detailed_timing* out);
+FS_API const char* fs_fine_section_name(int i); /* FINE_SECTIONS[i]; NULL out of range */
+FS_API const char* fs_cg_category_name(int i);  /* CG_CATS[i]; NULL out of range */
+/* GpuFluid::dispatches_per_substep (src/gpu/fluid.rs:1273-1276): kernel
+ * dispatches the last substep issued, counted as they were enqueued. */
+FS_API uint32_t fs_dispatches_per_substep(const fs_tank* t);
+/* Profiler::stats_json (src/profiler/mod.rs:416-594): the simulation's part
+ * of the object, with the reference's field names and number formatting.
+ * Timestep fields come from f (NULL: the profiler's defaults, zeros).  "gpu"
+ * is the GpuTimers readout: detailed when detailed timing is on, else coarse
+ * (fs_set_timing), else null.  Browser and render fields are not emitted.
+ * Writes a NUL-terminated string when cap > *len_out; with buf NULL only
+ * *len_out is set. */
+FS_API int fs_stats_json(fs_tank* t, const fs_frame* f, char* buf, size_t cap, size_t* len_out);
 
 /* ---- scene presets and particle seeding ----
  * The reference builds its initial particles from the registry on the host:
