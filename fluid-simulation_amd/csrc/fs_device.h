@@ -47,14 +47,21 @@ __device__ __forceinline__ int32_t sat_i32(float x) {
 
 // Cell key of sh/mark.wgsl:35-43: g = (pos - origin) / h (division), clamped
 // floor.  The same expression the sort keys use (sh/sort_scatter.wgsl:43-51).
-__device__ __forceinline__ uint32_t cell_key(const StepParams& P, float x, float y, float z) {
+// The clamped cell coordinates of a position and their row-major key.
+__device__ __forceinline__ uint32_t cell_ijk(const StepParams& P, float x, float y, float z, uint32_t& i,
+                                             uint32_t& j, uint32_t& k) {
     float gx = (x - P.origin[0]) / P.h;
     float gy = (y - P.origin[1]) / P.h;
     float gz = (z - P.origin[2]) / P.h;
-    uint32_t i = (uint32_t)clampf(floorf(gx), 0.0f, (float)(P.nx - 1u));
-    uint32_t j = (uint32_t)clampf(floorf(gy), 0.0f, (float)(P.ny - 1u));
-    uint32_t k = (uint32_t)clampf(floorf(gz), 0.0f, (float)(P.nz - 1u));
+    i = (uint32_t)clampf(floorf(gx), 0.0f, (float)(P.nx - 1u));
+    j = (uint32_t)clampf(floorf(gy), 0.0f, (float)(P.ny - 1u));
+    k = (uint32_t)clampf(floorf(gz), 0.0f, (float)(P.nz - 1u));
     return i + P.nx * (j + P.ny * k);
+}
+
+__device__ __forceinline__ uint32_t cell_key(const StepParams& P, float x, float y, float z) {
+    uint32_t i, j, k;
+    return cell_ijk(P, x, y, z, i, j, k);
 }
 
 __device__ __forceinline__ void face_dims(const StepParams& P, int a, uint32_t d[3]) {

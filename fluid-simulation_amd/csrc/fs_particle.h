@@ -170,8 +170,8 @@ __device__ __forceinline__ void p2g_atomic_particle(const StepParams& P, const P
 
 // Sort key of particle p plus whether its P2G cell differs from it.
 __device__ __forceinline__ uint32_t particle_key(const StepParams& P, float x, float y, float z, bool& misfit) {
-    const uint32_t key = cell_key(P, x, y, z);
-    const uint32_t ki = key % P.nx, kj = (key / P.nx) % P.ny, kk = key / (P.nx * P.ny);
+    uint32_t ki, kj, kk;
+    const uint32_t key = cell_ijk(P, x, y, z, ki, kj, kk);
     misfit = floorf((x - P.origin[0]) * P.inv_h) != (float)ki || floorf((y - P.origin[1]) * P.inv_h) != (float)kj ||
              floorf((z - P.origin[2]) * P.inv_h) != (float)kk;
     return key;
