@@ -66,8 +66,23 @@ def run_gpu(fsm, sc, params, steps, warmup, timing_steps=3):
     for _ in range(timing_steps):
         t.step(1)
         per.append(t.stats())
+    # One more substep with the reference's detailed sections (outside the
+    # timed region: the section events add a wait per substep).
+    t.set_timing(False)
+    t.set_detailed_timing(True)
+    t.step(1)
+    per[-1]["detailed"] = t.detailed_timing()
+    per[-1]["dispatches_per_substep"] = t.dispatches_per_substep()
     t.close()
     return dt, per
+
+
+def fine_sections(d):
+    """One substep's device time by the reference's FINE_SECTIONS names (the
+    fused engine's absorbed sections omitted) and its CG as one solve."""
+    out = {k: round(v, 4) for k, v in d["sections"].items() if k not in d["fused"]}
+    out["cg_solve"] = round(d["cg_total_ms"], 4)
+    return out
 
 
 def cpu_baseline(sc, steps):
@@ -581,6 +596,8 @@ def main():
         "cg": {"iterations_per_solve": iters, "active_cells": act, "ms_per_solve": cg_ms,
                "ms_per_iteration": [m / max(k, 1) for m, k in zip(cg_ms, iters)]},
         "sections_ms": {k: per[-1][k] for k in ("ms_prep", "ms_pressure", "ms_finish", "ms_cg")},
+        "fine_sections_ms": fine_sections(per[-1]["detailed"]),
+        "dispatches_per_substep": per[-1]["dispatches_per_substep"],
         "roofline": {
             "kernel": "k_cg_persistent",
             "bound": "hbm",
