@@ -452,6 +452,60 @@ def view_report(fsm, scenes, gpu_params, reps=20):
             "note": "pack timed as back-to-back calls on the tank's stream, host wall clock"}
 
 
+def default_scene_report(fsm, steps=240, warmup=24, frames=120):
+    """The reference's default configuration through the drop-in: registry
+    defaults (falling blob in an 80x40x80 tank, particles.density 10, 30 CG
+    iterations, sort every 4th substep), the tank built by
+    fs_create_from_settings (GpuFluid::new) and stepped by fs_step, for the
+    fused engine and for the reference-structured FP32 engine on the same
+    seeded particles; then the frame driver (fs_frame_run) at a 60 Hz render
+    delta, as FluidApp::frame runs it."""
+    r = fsm.Registry()
+    sc = fsm.scene_from_registry(r)
+    out = {"report": "default_scene", "grid": list(sc.res), "preset": int(sc.preset),
+           "particles": int(sc.generated_count), "pressure_iterations": int(r.get("solver.pressure_iterations")),
+           "sort_period": int(r.get("dev.particle_sort_period")), "engines": {}}
+    fused = fsm.FluidStep.from_settings(r, sc)
+    pos0, _ = fused.particles()
+    ref_params = fsm.Params.from_buffer_copy(fused.params)
+    ref_params.engine, ref_params.precision = fsm.ENGINE_REFERENCE, fsm.PRECISION_F32
+    tanks = {"fused": fused, "reference": fsm.FluidStep(ref_params, pos0)}
+    for name, t in tanks.items():
+        t.step(warmup)
+        t.synchronize()
+        t0 = time.perf_counter()
+        t.step(steps)
+        t.synchronize()
+        dt = time.perf_counter() - t0
+        st = t.stats()
+        # Four substeps (one sort period) with the reference's detailed sections.
+        t.set_detailed_timing(True)
+        t.step(4)
+        d = t.detailed_timing()
+        t.set_detailed_timing(False)
+        out["engines"][name] = {"particle_steps_per_s": t.n * steps / dt, "ms_per_substep": dt / steps * 1e3,
+                                "cg_iterations_last": st["cg_iterations"], "liquid_cells": st["liquid_cells"],
+                                "fine_sections_ms_4_substeps": fine_sections(d),
+                                "dispatches_per_substep": t.dispatches_per_substep()}
+        print(f"default scene: {name} done", file=sys.stderr, flush=True)
+    tanks["reference"].close()
+    f = fsm.Frame(fused, r)
+    f.run(16.6667)
+    fused.synchronize()
+    subs = 0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        subs += f.run(16.6667)["substeps"]
+    fused.synchronize()
+    dt = time.perf_counter() - t0
+    out["frame_driver"] = {"frames": frames, "render_dt_ms": 16.6667, "substeps": subs,
+                           "ms_per_frame": dt / frames * 1e3,
+                           "stats_json": json.loads(fused.stats_json(f))}
+    f.close()
+    fused.close()
+    return out
+
+
 def sections_report(fsm, scenes, gpu_params, steps=2):
     """Per-section device time of both engines on the 128^3 tank under the
     reference's FINE_SECTIONS / CG_CATS names (fs_get_detailed_timing), each
@@ -497,6 +551,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="adjoint report: skip the FP64 reference gradient")
     ap.add_argument("--view-report", action="store_true",
                     help="instead of timing the step, time the live-buffer hand-off and the vec4 particle pack")
+    ap.add_argument("--default-scene-report", action="store_true",
+                    help="instead of timing the step, step the reference's default scene built from the registry "
+                         "defaults, with both engines and through the frame driver")
     ap.add_argument("--sections-report", action="store_true",
                     help="instead of timing the step, report both engines' per-section device time under the "
                          "reference's section names")
@@ -533,6 +590,10 @@ def main():
 
     if args.scene_report:
         print(json.dumps(scene_report(fsm)), flush=True)
+        return
+
+    if args.default_scene_report:
+        print(json.dumps(default_scene_report(fsm)), flush=True)
         return
 
     if args.sections_report:

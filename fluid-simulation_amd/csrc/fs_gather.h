@@ -48,13 +48,15 @@ struct GatherOut {
 
 // Brick at cell (x0, y0, z0).  sp: the sorted side; src: the side the keys
 // were computed on (listed indices refer to it); [pbeg, pend): the particle
-// range of this tank in both sides.
+// range of this tank in both sides.  With perm, sp is not sorted: sorted slot
+// d holds particle perm[d] of sp (the particles were not moved).
 template <int BF16, int V, int THREADS = kGThreads>
 __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const ParticleSoA& sp, const ParticleSoA& src,
                                                  const uint32_t* __restrict__ occ, const uint32_t* __restrict__ off,
                                                  const uint8_t* __restrict__ ct, const GatherOut& out, int32_t x0,
                                                  int32_t y0, int32_t z0, const uint32_t* __restrict__ mis_list,
-                                                 uint32_t mis_n, uint32_t pbeg, uint32_t pend) {
+                                                 uint32_t mis_n, uint32_t pbeg, uint32_t pend,
+                                                 const uint32_t* __restrict__ perm = nullptr) {
     constexpr int kGBX = Brick<V>::X, kGBY = Brick<V>::Y, kGBZ = Brick<V>::Z;
     constexpr int kGRowsMax = (kGBY + 2) * (kGBZ + 2);
     constexpr int kGU = (kGBX + 1) * kGBY * kGBZ;
@@ -176,7 +178,8 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
         if (t >= total) continue;
         // Find the row holding flattened item t (rows are few; linear advance).
         while (s_row_pref[row + 1] <= t) ++row;
-        const uint32_t p = s_row_start[row] + (t - s_row_pref[row]);
+        const uint32_t d = s_row_start[row] + (t - s_row_pref[row]);
+        const uint32_t p = perm ? perm[d] : d;
         const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
                              load_vel<BF16>(sp.vel[2], p)};
         add_particle(sp.pos[0][p], sp.pos[1][p], sp.pos[2][p], pv);

@@ -302,6 +302,11 @@ int mark(fs_tank* t, int tag) {
         if (int rc_ = mark((t), (tag))) return rc_; \
     } while (0)
 
+bool env_is(const char* name, const char* value) {
+    const char* e = std::getenv(name);
+    return e && std::strcmp(e, value) == 0;
+}
+
 // Stable sort of the live side into the other side (both engines).
 int sort_particles(fs_tank* t) {
     const StepParams& P = t->P;
@@ -435,6 +440,25 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
                                  t->saved, t->counters + 1, t->mis_list, s, t->gather_brick));
         // Fallback when more particles' P2G cells differ from their sort
         // cells than the gather walks.
+        for (int a = 0; a < 3; ++a) {
+            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->num[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
+            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
+        }
+        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
+        FS_MARK(t, kSecScatter);
+        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
+        FS_MARK(t, kSecNormalize);
+        t->gather_last = 1;
+    } else if (t->gperm) {
+        // No sort this substep: the bin starts and the stable cell order of
+        // the live side give the gather its segments, and it reads the
+        // particles through that order where they lie.  The sums are the same
+        // integer terms as the atomic form's, so the result is identical; the
+        // particle order stays the reference's.
+        FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
+        FS_HIP(launch_stable_permutation(t->keys, P.N, P.C, t->ktmp0, t->ktmp1, t->vtmp, t->gperm, t->radix_ws, s));
+        FS_HIP(launch_p2g_gather(P, live, live, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
+                                 t->counters + 1, t->mis_list, s, t->gather_brick, t->gperm));
         for (int a = 0; a < 3; ++a) {
             FS_HIP(launch_fill_u32_guarded((uint32_t*)t->num[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
             FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
@@ -686,6 +710,9 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(t->alloc(t->ktmp1, n));
         FS_HIP(t->alloc(t->vtmp, n));
         FS_HIP(t->alloc(t->perm, n));
+        const bool unsorted_substeps = !q.particle_sort || q.particle_sort_period > 1;
+        if (q.engine == FS_ENGINE_FUSED && unsorted_substeps && !env_is("FLUIDSTEP_UNSORTED_P2G", "atomic"))
+            FS_HIP(t->alloc(t->gperm, n));
         FS_HIP(t->alloc(t->radix_ws, radix_workspace(n)));
         FS_HIP(t->alloc(t->scan_ws, scan_workspace(C) + scan_workspace(C / 64 + 1)));
         FS_HIP(t->alloc(t->occ, C));

@@ -84,11 +84,14 @@ hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const ui
 // Sorted-segment P2G gather per brick of cells, with the face epilogue fused.
 // brick_variant: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells per workgroup.
 // src is the pre-sort side, which the listed misfit indices refer to; the
-// gather does nothing when *mis_count exceeds kMisfitCap (fs_gather.h).
+// gather does nothing when *mis_count exceeds kMisfitCap (fs_gather.h).  With
+// perm (the stable cell order of `sorted`, whose particles were not moved),
+// sorted slot d is particle perm[d].
 hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
                              const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct,
                              float* const vel[3], float* const saved[3], const uint32_t* mis_count,
-                             const uint32_t* mis_list, hipStream_t s, int brick_variant = 0);
+                             const uint32_t* mis_list, hipStream_t s, int brick_variant = 0,
+                             const uint32_t* perm = nullptr);
 
 // ---- k_cg.hip: pressure solve ----
 // Reference-structured CG passes (sh/cg_*.wgsl), f32 scalars in `scal`.

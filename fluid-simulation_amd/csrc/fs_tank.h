@@ -30,6 +30,9 @@ struct fs_tank {
     int cur = 0;
     bool sort_swapped = false;  // take_sort_swapped: the live side flipped since the last query
     uint32_t *keys = nullptr, *ktmp0 = nullptr, *ktmp1 = nullptr, *vtmp = nullptr, *perm = nullptr;
+    // Fused engine, substeps without a sort: the live side's stable cell order,
+    // through which the gather reads the particles (they stay in place).
+    uint32_t* gperm = nullptr;
     uint32_t *radix_ws = nullptr, *scan_ws = nullptr;
     uint32_t *occ = nullptr, *cell_offset = nullptr;
     uint8_t* ct = nullptr;

@@ -227,37 +227,39 @@ __global__ __launch_bounds__(THREADS) void k_p2g_gather(StepParams P, ParticleSo
                                                            const uint8_t* __restrict__ ct, GatherOut out,
                                                            uint32_t nbx, uint32_t nby, uint32_t nbz,
                                                            const uint32_t* __restrict__ mis_count,
-                                                           const uint32_t* __restrict__ mis_list) {
+                                                           const uint32_t* __restrict__ mis_list,
+                                                           const uint32_t* __restrict__ perm) {
     const uint32_t mis_n = *mis_count;
     if (mis_n > kMisfitCap) return;
     const uint32_t G = nbx * nby * nbz;
     const uint32_t b = xcd_remap(blockIdx.x, G);
     const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
     p2g_gather_brick<BF16, V, THREADS>(P, sp, src, occ, off, ct, out, (int32_t)bxi * Brick<V>::X, (int32_t)byi * Brick<V>::Y,
-                              (int32_t)bzi * Brick<V>::Z, mis_list, mis_n, 0u, P.N);
+                              (int32_t)bzi * Brick<V>::Z, mis_list, mis_n, 0u, P.N, perm);
 }
 
 namespace {
 template <int V>
 hipError_t launch_gather_v(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
                            const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct, const GatherOut& out,
-                           const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s) {
+                           const uint32_t* mis_count, const uint32_t* mis_list, const uint32_t* perm,
+                           hipStream_t s) {
     const uint32_t nbx = (P.nx + Brick<V>::X - 1) / Brick<V>::X, nby = (P.ny + Brick<V>::Y - 1) / Brick<V>::Y,
                    nbz = (P.nz + Brick<V>::Z - 1) / Brick<V>::Z;
     const uint32_t G = nbx * nby * nbz;
     if (G <= kGSmallGrid) {
         if (bf16)
             FS_LAUNCH((k_p2g_gather<1, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
-                               occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+                               occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list, perm);
         else
             FS_LAUNCH((k_p2g_gather<0, V, kGThreadsSmall>), dim3(G), dim3(kGThreadsSmall), 0, s, P, sorted, src,
-                               occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+                               occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list, perm);
     } else if (bf16) {
         FS_LAUNCH((k_p2g_gather<1, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
-                           occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+                           occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list, perm);
     } else {
         FS_LAUNCH((k_p2g_gather<0, V, kGThreadsLarge>), dim3(G), dim3(kGThreadsLarge), 0, s, P, sorted, src,
-                           occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list);
+                           occ, cell_offset, ct, out, nbx, nby, nbz, mis_count, mis_list, perm);
     }
     return hipGetLastError();
 }
@@ -266,12 +268,15 @@ hipError_t launch_gather_v(const StepParams& P, const ParticleSoA& sorted, const
 hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
                              const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct, float* const vel[3],
                              float* const saved[3], const uint32_t* mis_count, const uint32_t* mis_list,
-                             hipStream_t s, int brick_variant) {
+                             hipStream_t s, int brick_variant, const uint32_t* perm) {
     const GatherOut out = {{vel[0], vel[1], vel[2]}, {saved[0], saved[1], saved[2]}};
     switch (brick_variant) {
-        case 0: return launch_gather_v<0>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, s);
-        case 1: return launch_gather_v<1>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, s);
-        case 2: return launch_gather_v<2>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, s);
+        case 0: return launch_gather_v<0>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, perm,
+                                              s);
+        case 1: return launch_gather_v<1>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, perm,
+                                              s);
+        case 2: return launch_gather_v<2>(P, sorted, src, bf16, occ, cell_offset, ct, out, mis_count, mis_list, perm,
+                                              s);
         default: return hipErrorInvalidValue;
     }
 }

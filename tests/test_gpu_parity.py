@@ -381,3 +381,39 @@ def test_barrier_forms_give_identical_bits(scene_name, forms, blocks, monkeypatc
         assert it == iters, form
         for i, (a, b) in enumerate(zip(base, arrs)):
             assert bits_equal(a, b), (form, i)
+
+
+@pytest.mark.parametrize("n,seed,sort,period", [(20, 4, 1, 4), (24, 11, 1, 3), (16, 3, 0, 1)])
+def test_unsorted_substeps_gather_equals_atomic(monkeypatch, n, seed, sort, period):
+    """Substeps without a sort (the reference's default sorts every 4th): the
+    fused engine's gather reads the particles through the live side's stable
+    cell order, and must give the bits of the atomic P2G form
+    (FLUIDSTEP_UNSORTED_P2G=atomic), with escaped and misfit particles in the
+    tank, and keep the particle order (no move)."""
+    sc = scenes.small_tank(n=n, seed=seed, fill=0.45)
+    pos = sc["pos"].copy()
+    pos[0] = (-0.3, 0.5, 0.5)
+    pos[1] = (1.7, 1.2, -0.1)
+    if n == 24:
+        pos[5, 1] = _brick_edge_misfit_24()
+    p = gpu_params(sc, fsm.ENGINE_FUSED, particle_sort=sort, particle_sort_period=period)
+    say("unsorted gather", sc["name"], "sort", sort, "period", period)
+    monkeypatch.setenv("FLUIDSTEP_UNSORTED_P2G", "atomic")
+    a = fsm.FluidStep(p, pos, sc["vel"])
+    monkeypatch.delenv("FLUIDSTEP_UNSORTED_P2G")
+    g = fsm.FluidStep(p, pos, sc["vel"])
+    for _ in range(5):
+        a.step(1)
+        g.step(1)
+        sa, sg = a.stats(), g.stats()
+        assert sa["sorted"] == sg["sorted"]
+        if not sg["sorted"]:
+            assert sa["p2g_gather"] == 0 and sg["p2g_gather"] == 1
+        for k in (fsm.FACE_VELOCITY, fsm.FACE_SAVED):
+            fa, fg = a.faces(k), g.faces(k)
+            assert all(bits_equal(fa[i], fg[i]) for i in range(3))
+        assert bits_equal(a.pressure(), g.pressure())
+        pa, va = a.particles()
+        pg, vg = g.particles()
+        assert bits_equal(pa, pg) and bits_equal(va, vg)
+    a.close(), g.close()
