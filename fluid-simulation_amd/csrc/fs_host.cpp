@@ -440,10 +440,7 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
                                  t->saved, t->counters + 1, t->mis_list, s, t->gather_brick));
         // Fallback when more particles' P2G cells differ from their sort
         // cells than the gather walks.
-        for (int a = 0; a < 3; ++a) {
-            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->num[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
-            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
-        }
+        FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
         FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
         FS_MARK(t, kSecScatter);
         FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
@@ -459,10 +456,7 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
         FS_HIP(launch_stable_permutation(t->keys, P.N, P.C, t->ktmp0, t->ktmp1, t->vtmp, t->gperm, t->radix_ws, s));
         FS_HIP(launch_p2g_gather(P, live, live, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
                                  t->counters + 1, t->mis_list, s, t->gather_brick, t->gperm));
-        for (int a = 0; a < 3; ++a) {
-            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->num[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
-            FS_HIP(launch_fill_u32_guarded((uint32_t*)t->den[a], 0u, P.F[a], t->counters + 1, kMisfitCap, s));
-        }
+        FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
         FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
         FS_MARK(t, kSecScatter);
         FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));

@@ -60,6 +60,9 @@ hipError_t launch_classify(const StepParams& P, const uint32_t* occ, uint8_t* ct
                            hipStream_t s);
 // sh/scatter.wgsl: per-particle i32 atomics into num/den.
 // guard (nullable): run only when *guard > above.
+// The six num/den arrays cleared when *guard > above (fused fallback).
+hipError_t launch_clear_faces_guarded(const StepParams& P, int32_t* const num[3], int32_t* const den[3],
+                                      const uint32_t* guard, uint32_t above, hipStream_t s);
 hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf16,
                              int32_t* const num[3], int32_t* const den[3], const uint32_t* guard,
                              uint32_t above, hipStream_t s);

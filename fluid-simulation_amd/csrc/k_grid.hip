@@ -74,6 +74,32 @@ hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf
     return hipGetLastError();
 }
 
+// Clears num and den of the three axes when *guard > above (the fused
+// engine's fallback to the atomic P2G): one launch for the six arrays.
+__global__ void k_clear_faces_guarded(StepParams P, int32_t* __restrict__ n0, int32_t* __restrict__ n1,
+                                      int32_t* __restrict__ n2, int32_t* __restrict__ d0, int32_t* __restrict__ d1,
+                                      int32_t* __restrict__ d2, const uint32_t* __restrict__ guard, uint32_t above) {
+    if (*guard <= above) return;
+    const uint32_t tot = P.F[0] + P.F[1] + P.F[2];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < tot; gi += stride) {
+        int a;
+        uint32_t f;
+        split_face(P, gi, a, f);
+        (a == 0 ? n0 : (a == 1 ? n1 : n2))[f] = 0;
+        (a == 0 ? d0 : (a == 1 ? d1 : d2))[f] = 0;
+    }
+}
+
+hipError_t launch_clear_faces_guarded(const StepParams& P, int32_t* const num[3], int32_t* const den[3],
+                                      const uint32_t* guard, uint32_t above, hipStream_t s) {
+    const size_t tot = (size_t)P.F[0] + P.F[1] + P.F[2];
+    const unsigned need = grid_for(tot, 256), g = need < 1024u ? need : 1024u;
+    FS_LAUNCH(k_clear_faces_guarded, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
+              guard, above);
+    return hipGetLastError();
+}
+
 // ---- reference-structured per-axis passes ----
 __global__ void k_normalize(uint32_t n, const int32_t* __restrict__ num, const int32_t* __restrict__ den,
                             float* __restrict__ vel) {
@@ -188,7 +214,9 @@ hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3]
                                 uint32_t above, hipStream_t s) {
     const size_t tot = (size_t)P.F[0] + P.F[1] + P.F[2];
     unsigned g = grid_for(tot, 256);
-    if (guard && g > 4096) g = 4096;  // usually a no-op fallback: small grid-stride grid
+    // Usually a no-op fallback: a grid-stride grid small enough to be resident
+    // at once, so the idle launch is one round of waves reading the guard.
+    if (guard && g > 1024) g = 1024;
     FS_LAUNCH(k_face_epilogue, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
                        ct, vel[0], vel[1], vel[2], saved[0], saved[1], saved[2], guard, above);
     return hipGetLastError();
