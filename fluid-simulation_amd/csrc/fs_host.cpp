@@ -447,13 +447,13 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
         FS_MARK(t, kSecNormalize);
         t->gather_last = 1;
     } else if (t->gperm) {
-        // No sort this substep: the bin starts and the stable cell order of
-        // the live side give the gather its segments, and it reads the
-        // particles through that order where they lie.  The sums are the same
-        // integer terms as the atomic form's, so the result is identical; the
-        // particle order stays the reference's.
+        // No sort this substep: the bin starts and a cell order of the live
+        // side give the gather its segments, and it reads the particles
+        // through that order where they lie.  The sums are the same integer
+        // terms as the atomic form's, in any order, so the result is
+        // identical; the particle order stays the reference's.
         FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
-        FS_HIP(launch_stable_permutation(t->keys, P.N, P.C, t->ktmp0, t->ktmp1, t->vtmp, t->gperm, t->radix_ws, s));
+        FS_HIP(launch_cell_order(t->keys, P.N, t->cell_offset, t->cursor, P.C, t->gperm, s));
         FS_HIP(launch_p2g_gather(P, live, live, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
                                  t->counters + 1, t->mis_list, s, t->gather_brick, t->gperm));
         FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
@@ -705,8 +705,10 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(t->alloc(t->vtmp, n));
         FS_HIP(t->alloc(t->perm, n));
         const bool unsorted_substeps = !q.particle_sort || q.particle_sort_period > 1;
-        if (q.engine == FS_ENGINE_FUSED && unsorted_substeps && !env_is("FLUIDSTEP_UNSORTED_P2G", "atomic"))
+        if (q.engine == FS_ENGINE_FUSED && unsorted_substeps && !env_is("FLUIDSTEP_UNSORTED_P2G", "atomic")) {
             FS_HIP(t->alloc(t->gperm, n));
+            FS_HIP(t->alloc(t->cursor, C));
+        }
         FS_HIP(t->alloc(t->radix_ws, radix_workspace(n)));
         FS_HIP(t->alloc(t->scan_ws, scan_workspace(C) + scan_workspace(C / 64 + 1)));
         FS_HIP(t->alloc(t->occ, C));

@@ -48,6 +48,10 @@ size_t radix_workspace(size_t n);  // u32 elements (histogram table + scan ws)
 // Stable LSD radix sort of keys[0..n) (keys < key_limit); writes perm[d] = the
 // source index of sorted slot d.  keys_tmp/vals_tmp are n-sized scratch.
 // tile_variant picks the keys per workgroup: 0 = 4096, 1 = 2048, 2 = 8192.
+// order: the particles grouped by cell in key order (not stable), from the
+// keys, their bin starts and a scratch cursor of ncells words.
+hipError_t launch_cell_order(const uint32_t* keys, uint32_t n, const uint32_t* cell_offset, uint32_t* cursor,
+                             uint32_t ncells, uint32_t* order, hipStream_t s);
 hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t key_limit,
                                      uint32_t* keys_tmp0, uint32_t* keys_tmp1, uint32_t* vals_tmp,
                                      uint32_t* perm, uint32_t* ws, hipStream_t s, int tile_variant = 0);

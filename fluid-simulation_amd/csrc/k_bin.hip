@@ -117,6 +117,44 @@ hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint3
     return hipGetLastError();
 }
 
+// Cell order without stability: particle p takes slot off[key] + (a count of
+// the particles of its cell placed before it), one atomic per run of equal
+// keys among a wave's lanes.  The slots of cell k are exactly
+// [off[k], off[k] + occ[k]) when occ counted the same keys, so `order` is a
+// permutation grouping the particles by cell in key order; the order inside
+// a cell is whatever the atomics give.
+__global__ void k_cell_order(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ off,
+                             uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n;
+    const uint32_t key = valid ? keys[p] : 0xffffffffu;
+    const uint32_t lane = lane_id();
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t act = __ballot(valid);
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)(64 - __builtin_clzll(act));
+        base = atomicAdd(&cursor[key], end - lane);
+    }
+    // The head of this lane's run: the highest head lane at or below it.
+    const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t hl = upto ? 63u - (uint32_t)__builtin_clzll(upto) : 0u;
+    base = __shfl(base, (int)hl, 64);
+    if (valid) order[off[key] + base + (lane - hl)] = p;
+}
+
+hipError_t launch_cell_order(const uint32_t* keys, uint32_t n, const uint32_t* cell_offset, uint32_t* cursor,
+                             uint32_t ncells, uint32_t* order, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(cursor, 0u, ncells, s);
+    if (e != hipSuccess) return e;
+    FS_LAUNCH(k_cell_order, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, cell_offset, cursor, order);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive scan (u32, wrapping adds): tiles of 2048, tile sums scanned
 // recursively, then added back.  Integer prefix sums have one answer, so the
