@@ -307,13 +307,16 @@ bool env_is(const char* name, const char* value) {
     return e && std::strcmp(e, value) == 0;
 }
 
-// Stable sort of the live side into the other side (both engines).
-int sort_particles(fs_tank* t) {
+// Stable sort of the live side into the other side (both engines).  With
+// move = false the permutation is computed but not applied: the particles
+// stay on what becomes the other side, and the caller fills the new live
+// side through t->perm.
+int sort_particles(fs_tank* t, bool move = true) {
     const StepParams& P = t->P;
     hipStream_t s = t->stream;
     FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
     FS_HIP(launch_stable_permutation(t->keys, P.N, P.C, t->ktmp0, t->ktmp1, t->vtmp, t->perm, t->radix_ws, s));
-    FS_HIP(launch_permute(P.N, t->perm, t->side[t->cur], t->side[t->cur ^ 1], t->bf16, s));
+    if (move) FS_HIP(launch_permute(P.N, t->perm, t->side[t->cur], t->side[t->cur ^ 1], t->bf16, s));
     t->cur ^= 1;
     t->sort_swapped = true;  // src/gpu/fluid.rs:1570
     t->perm_valid = 1;
@@ -428,20 +431,23 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     FS_HIP(launch_key_hist(P, t->side[t->cur].pos, t->keys, t->occ, t->counters + 1, t->mis_list, kMisfitCap, s));
     FS_MARK(t, kSecMark);
     if (sort_now) {
-        int rc = sort_particles(t);
+        // The permutation is applied by G2P, which rewrites every particle
+        // anyway; until then the particles lie on the pre-sort side.
+        int rc = sort_particles(t, false);
         if (rc != FS_OK) return rc;
     }
     FS_MARK(t, kSecSort);
     ParticleSoA live = t->side[t->cur];
+    const ParticleSoA pre = t->side[t->cur ^ 1];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
     FS_MARK(t, kSecClassify);
     if (sort_now) {
-        FS_HIP(launch_p2g_gather(P, live, t->side[t->cur ^ 1], t->bf16, t->occ, t->cell_offset, t->ct, t->vel,
-                                 t->saved, t->counters + 1, t->mis_list, s, t->gather_brick));
+        FS_HIP(launch_p2g_gather(P, pre, pre, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
+                                 t->counters + 1, t->mis_list, s, t->gather_brick, t->perm));
         // Fallback when more particles' P2G cells differ from their sort
-        // cells than the gather walks.
+        // cells than the gather walks (any particle order will do).
         FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
-        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
+        FS_HIP(launch_p2g_atomic(P, pre, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
         FS_MARK(t, kSecScatter);
         FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
         FS_MARK(t, kSecNormalize);
@@ -510,8 +516,14 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     if (t->timing) FS_HIP(hipEventRecord(t->ev[2], s));
     FS_HIP(launch_gradient_enforce(P, t->p, t->ct, t->vel, s));
     FS_MARK(t, kSecGradient);
-    if (stop_before_g2p) return FS_OK;
-    FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
+    if (stop_before_g2p) {
+        if (sort_now) FS_HIP(launch_permute(P.N, t->perm, pre, live, t->bf16, s));
+        return FS_OK;
+    }
+    if (sort_now)
+        FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s, &pre, t->perm));
+    else
+        FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
     FS_MARK(t, kSecG2p);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[3], s));
     return FS_OK;

@@ -191,8 +191,11 @@ hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* 
                            float* d0, float* d1, float* q, const CgWork& w, int grid_blocks, hipStream_t s);
 
 // ---- k_g2p.hip ----
+// With from and perm (both or neither), particle d of live is read from
+// slot perm[d] of *from, i.e. the sort's permutation is applied as G2P writes.
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16,
-                      const float* const vel[3], const float* const saved[3], hipStream_t s);
+                      const float* const vel[3], const float* const saved[3], hipStream_t s,
+                      const ParticleSoA* from = nullptr, const uint32_t* perm = nullptr);
 // The reference's 32-byte particle record {pos.xyz, 0, vel.xyz, 0} of the
 // side `sp` into dst (8n floats), and u8 -> u32 widening (k_view.hip).
 hipError_t launch_pack_particles_vec4(const ParticleSoA& sp, uint32_t n, bool bf16, float* dst, hipStream_t s);

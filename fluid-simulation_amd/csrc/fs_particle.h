@@ -84,14 +84,16 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
 
 // G2P + FLIP/PIC blend + CFL clamp + wall friction + advection + wall clamp
 // for particle p (sh/g2p.wgsl:131-198), in place.
+// The particle is read from slot q of rd and written to slot p of sp.
 template <int BF16>
-__device__ __forceinline__ void g2p_particle(const StepParams& P, const ParticleSoA& sp, uint32_t p,
-                                             const float* __restrict__ u, const float* __restrict__ v,
-                                             const float* __restrict__ w, const float* __restrict__ us,
-                                             const float* __restrict__ vs, const float* __restrict__ ws) {
+__device__ __forceinline__ void g2p_particle_from(const StepParams& P, const ParticleSoA& rd, uint32_t q,
+                                                  const ParticleSoA& sp, uint32_t p, const float* __restrict__ u,
+                                                  const float* __restrict__ v, const float* __restrict__ w,
+                                                  const float* __restrict__ us, const float* __restrict__ vs,
+                                                  const float* __restrict__ ws) {
     const float h = P.h, dt = P.dt, alpha = P.flip;
-    const float Pp[3] = {sp.pos[0][p], sp.pos[1][p], sp.pos[2][p]};
-    const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p), load_vel<BF16>(sp.vel[2], p)};
+    const float Pp[3] = {rd.pos[0][q], rd.pos[1][q], rd.pos[2][q]};
+    const float pv[3] = {load_vel<BF16>(rd.vel[0], q), load_vel<BF16>(rd.vel[1], q), load_vel<BF16>(rd.vel[2], q)};
     float pic[3], sav[3];
     sample_axis(P, 0, Pp, u, us, pic[0], sav[0]);
     sample_axis(P, 1, Pp, v, vs, pic[1], sav[1]);
@@ -144,6 +146,14 @@ __device__ __forceinline__ void g2p_particle(const StepParams& P, const Particle
         sp.pos[a][p] = np[a];
         store_vel<BF16>(sp.vel[a], p, nv[a]);
     }
+}
+
+template <int BF16>
+__device__ __forceinline__ void g2p_particle(const StepParams& P, const ParticleSoA& sp, uint32_t p,
+                                             const float* __restrict__ u, const float* __restrict__ v,
+                                             const float* __restrict__ w, const float* __restrict__ us,
+                                             const float* __restrict__ vs, const float* __restrict__ ws) {
+    g2p_particle_from<BF16>(P, sp, p, sp, p, u, v, w, us, vs, ws);
 }
 
 // One particle of sh/scatter.wgsl:78-118 with global i32 atomics.

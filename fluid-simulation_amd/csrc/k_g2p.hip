@@ -13,8 +13,12 @@ namespace {
 inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 }
 
+// With perm, particle p of sp is read from slot perm[p] of rd (a sort whose
+// permutation was not applied yet): G2P moves the particles as it updates
+// them.
 template <int BF16>
-__global__ __launch_bounds__(256) void k_g2p(StepParams P, ParticleSoA sp, const float* __restrict__ u,
+__global__ __launch_bounds__(256) void k_g2p(StepParams P, ParticleSoA sp, ParticleSoA rd,
+                                              const uint32_t* __restrict__ perm, const float* __restrict__ u,
                                               const float* __restrict__ v, const float* __restrict__ w,
                                               const float* __restrict__ us, const float* __restrict__ vs,
                                               const float* __restrict__ ws) {
@@ -22,19 +26,21 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, ParticleSoA sp, const
     // lines a region of particles reads are fetched into one XCD's L2.
     const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= P.N) return;
-    g2p_particle<BF16>(P, sp, p, u, v, w, us, vs, ws);
+    g2p_particle_from<BF16>(P, rd, perm ? perm[p] : p, sp, p, u, v, w, us, vs, ws);
 }
 
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16, const float* const vel[3],
-                      const float* const saved[3], hipStream_t s) {
+                      const float* const saved[3], hipStream_t s, const ParticleSoA* from, const uint32_t* perm) {
     if (P.N == 0) return hipSuccess;
+    if ((from == nullptr) != (perm == nullptr)) return hipErrorInvalidValue;
+    const ParticleSoA rd = from ? *from : live;
     const unsigned G = (grid_for(P.N, 256) + 7u) / 8u * 8u;  // whole XCD groups for xcd_remap
     if (bf16)
-        FS_LAUNCH(k_g2p<1>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
-                           saved[2]);
+        FS_LAUNCH(k_g2p<1>, dim3(G), dim3(256), 0, s, P, live, rd, perm, vel[0], vel[1], vel[2], saved[0],
+                  saved[1], saved[2]);
     else
-        FS_LAUNCH(k_g2p<0>, dim3(G), dim3(256), 0, s, P, live, vel[0], vel[1], vel[2], saved[0], saved[1],
-                           saved[2]);
+        FS_LAUNCH(k_g2p<0>, dim3(G), dim3(256), 0, s, P, live, rd, perm, vel[0], vel[1], vel[2], saved[0],
+                  saved[1], saved[2]);
     return hipGetLastError();
 }
 
