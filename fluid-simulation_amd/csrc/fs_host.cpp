@@ -441,29 +441,26 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     const ParticleSoA pre = t->side[t->cur ^ 1];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
     FS_MARK(t, kSecClassify);
-    if (sort_now) {
-        FS_HIP(launch_p2g_gather(P, pre, pre, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
-                                 t->counters + 1, t->mis_list, s, t->gather_brick, t->perm));
+    if (sort_now || t->gperm) {
+        // The gather walks the particles by cell through an order of them.
+        // On a sort substep they still lie on the pre-sort side, in the
+        // sort's order (perm), which G2P applies as it writes.  Otherwise they
+        // lie on the live side and stay there; a cell order of them (gperm,
+        // not stable) serves, since the gather's integer sums are the atomic
+        // form's terms in any order.
+        const ParticleSoA& from = sort_now ? pre : live;
+        const uint32_t* order = t->perm;
+        if (!sort_now) {
+            FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
+            FS_HIP(launch_cell_order(t->keys, P.N, t->cell_offset, t->cursor, P.C, t->gperm, s));
+            order = t->gperm;
+        }
+        FS_HIP(launch_p2g_gather(P, from, from, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
+                                 t->counters + 1, t->mis_list, s, t->gather_brick, order));
         // Fallback when more particles' P2G cells differ from their sort
         // cells than the gather walks (any particle order will do).
         FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
-        FS_HIP(launch_p2g_atomic(P, pre, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
-        FS_MARK(t, kSecScatter);
-        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
-        FS_MARK(t, kSecNormalize);
-        t->gather_last = 1;
-    } else if (t->gperm) {
-        // No sort this substep: the bin starts and a cell order of the live
-        // side give the gather its segments, and it reads the particles
-        // through that order where they lie.  The sums are the same integer
-        // terms as the atomic form's, in any order, so the result is
-        // identical; the particle order stays the reference's.
-        FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
-        FS_HIP(launch_cell_order(t->keys, P.N, t->cell_offset, t->cursor, P.C, t->gperm, s));
-        FS_HIP(launch_p2g_gather(P, live, live, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
-                                 t->counters + 1, t->mis_list, s, t->gather_brick, t->gperm));
-        FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
-        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
+        FS_HIP(launch_p2g_atomic(P, from, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
         FS_MARK(t, kSecScatter);
         FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
         FS_MARK(t, kSecNormalize);
