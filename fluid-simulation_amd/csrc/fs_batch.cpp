@@ -192,18 +192,19 @@ int enqueue_substep(fs_batch* bt, int cur, hipStream_t s) {
     FS_HIP(launch_exclusive_scan(bt->occ, bt->cell_offset, bt->C_tot, bt->scan_ws, s));
     FS_HIP(launch_stable_permutation(bt->keys, bt->N_tot, bt->C_tot, bt->ktmp0, bt->ktmp1, bt->vtmp, bt->perm,
                                      bt->radix_ws, s, (int)bt->tuning.sort_tile));
-    FS_HIP(launch_permute(bt->N_tot, bt->perm, src, dst, bt->bf16, s));
+    // The permutation is applied by G2P, which rewrites every particle; until
+    // then the gather reads the particles on src through it.
     FS_HIP(launch_batch_classify(bt->d_tanks, bt->wl_cell, bt->n_cell, bt->occ, bt->ct, bt->counters, s));
     const int v = (int)bt->tuning.gather_brick;
-    FS_HIP(launch_batch_gather(bt->d_tanks, bt->wl_brick[v], bt->n_brick[v], v, bt->bf16, dst, src, bt->occ,
-                               bt->cell_offset, bt->ct, bt->grid, bt->mis_count(), bt->mis_list, s));
+    FS_HIP(launch_batch_gather(bt->d_tanks, bt->wl_brick[v], bt->n_brick[v], v, bt->bf16, src, src, bt->occ,
+                               bt->cell_offset, bt->ct, bt->grid, bt->mis_count(), bt->mis_list, s, bt->perm));
     // Atomic form when more particles' P2G cells differ from their sort cells
     // than the gather walks (every launch below is a no-op otherwise).
     for (int a = 0; a < 3; ++a) {
         FS_HIP(launch_fill_u32_guarded((uint32_t*)bt->grid.num[a], 0u, bt->F_tot[a], bt->mis_count(), kMisfitCap, s));
         FS_HIP(launch_fill_u32_guarded((uint32_t*)bt->grid.den[a], 0u, bt->F_tot[a], bt->mis_count(), kMisfitCap, s));
     }
-    FS_HIP(launch_batch_p2g_fallback(bt->d_tanks, bt->wl_part, bt->n_part, bt->wl_face, bt->n_face, bt->bf16, dst,
+    FS_HIP(launch_batch_p2g_fallback(bt->d_tanks, bt->wl_part, bt->n_part, bt->wl_face, bt->n_face, bt->bf16, src,
                                      bt->ct, bt->grid, bt->mis_count(), s));
     FS_HIP(launch_batch_div_cginit(bt->d_tanks, bt->wl_cell, bt->n_cell, bt->grid, bt->ct, bt->occ, bt->div, bt->b,
                                    bt->p, bt->r, bt->d0, bt->cg, s));
@@ -215,7 +216,7 @@ int enqueue_substep(fs_batch* bt, int cur, hipStream_t s) {
     FS_HIP(launch_batch_cg(bt->d_tanks, bt->wl_cg, bt->n_cg, (int)bt->tuning.cg_unroll, bt->b, bt->p, bt->r, bt->d0,
                            bt->d1, bt->q, bt->cg, s));
     FS_HIP(launch_batch_gradient_enforce(bt->d_tanks, bt->wl_face, bt->n_face, bt->p, bt->ct, bt->grid, s));
-    FS_HIP(launch_batch_g2p(bt->d_tanks, bt->wl_part, bt->n_part, bt->bf16, dst, bt->grid, s));
+    FS_HIP(launch_batch_g2p(bt->d_tanks, bt->wl_part, bt->n_part, bt->bf16, dst, bt->grid, s, &src, bt->perm));
     return FS_OK;
 }
 
@@ -583,8 +584,8 @@ FS_API int fs_batch_autotune(fs_batch* bt, fs_batch_tuning* out) {
     for (int v = 0; v < 3; ++v) {
         double ms = 0.0;
         auto run = [&]() -> int {
-            FS_HIP(launch_batch_gather(bt->d_tanks, bt->wl_brick[v], bt->n_brick[v], v, bt->bf16, dst, src, bt->occ,
-                                       bt->cell_offset, bt->ct, bt->grid, bt->mis_count(), bt->mis_list, s));
+            FS_HIP(launch_batch_gather(bt->d_tanks, bt->wl_brick[v], bt->n_brick[v], v, bt->bf16, src, src, bt->occ,
+                                       bt->cell_offset, bt->ct, bt->grid, bt->mis_count(), bt->mis_list, s, bt->perm));
             return FS_OK;
         };
         if (int rc = run()) return rc;

@@ -70,7 +70,8 @@ hipError_t launch_batch_classify(const BatchTank* T, const WorkItem* wl, uint32_
 hipError_t launch_batch_gather(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int brick_variant, int bf16,
                                const ParticleSoA& sorted, const ParticleSoA& src, const uint32_t* occ,
                                const uint32_t* cell_offset, const uint8_t* ct, const BatchGrid& g,
-                               const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s);
+                               const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s,
+                               const uint32_t* perm = nullptr);  // with perm: slot d of `sorted` is perm[d]
 // Atomic P2G + face epilogue, run only when *mis_count > kMisfitCap (num/den
 // cleared beforehand by guarded fills).
 hipError_t launch_batch_p2g_fallback(const BatchTank* T, const WorkItem* wl_part, uint32_t npart,
@@ -87,7 +88,10 @@ hipError_t launch_batch_cg(const BatchTank* T, const WorkItem* wl, uint32_t nwl,
 int batch_cg_max_blocks();
 hipError_t launch_batch_gradient_enforce(const BatchTank* T, const WorkItem* wl, uint32_t nwl, const float* p,
                                          const uint8_t* ct, const BatchGrid& g, hipStream_t s);
+// With from and perm, particle d of sp is read from slot perm[d] of *from
+// (the sort's permutation applied as G2P writes).
 hipError_t launch_batch_g2p(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int bf16, const ParticleSoA& sp,
-                            const BatchGrid& g, hipStream_t s);
+                            const BatchGrid& g, hipStream_t s, const ParticleSoA* from = nullptr,
+                            const uint32_t* perm = nullptr);
 
 }  // namespace fs

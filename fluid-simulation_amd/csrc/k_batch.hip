@@ -99,7 +99,8 @@ __global__ __launch_bounds__(kGThreads) void kb_gather(const BatchTank* __restri
                                                         const uint32_t* __restrict__ off,
                                                         const uint8_t* __restrict__ ct, BatchGrid g,
                                                         const uint32_t* __restrict__ mis_count,
-                                                        const uint32_t* __restrict__ mis_list) {
+                                                        const uint32_t* __restrict__ mis_list,
+                                                        const uint32_t* __restrict__ perm) {
     const uint32_t mis_n = *mis_count;
     if (mis_n > kMisfitCap) return;  // the atomic form handles this substep
     const WorkItem wi = wl[blockIdx.x];
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kGThreads) void kb_gather(const BatchTank* __restri
                            {g.saved[0] + bt.face_off[0], g.saved[1] + bt.face_off[1], g.saved[2] + bt.face_off[2]}};
     p2g_gather_brick<BF16, V>(P, sp, src, occ + co, off + co, ct + co, out, (int32_t)bxi * Brick<V>::X,
                               (int32_t)byi * Brick<V>::Y, (int32_t)bzi * Brick<V>::Z, mis_list, mis_n, part_off,
-                              part_off + P.N);
+                              part_off + P.N, perm);
 }
 
 namespace {
@@ -122,13 +123,14 @@ template <int V>
 hipError_t launch_batch_gather_v(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int bf16,
                                  const ParticleSoA& sorted, const ParticleSoA& src, const uint32_t* occ,
                                  const uint32_t* off, const uint8_t* ct, const BatchGrid& g,
-                                 const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s) {
+                                 const uint32_t* mis_count, const uint32_t* mis_list, const uint32_t* perm,
+                                 hipStream_t s) {
     if (bf16)
         hipLaunchKernelGGL((kb_gather<1, V>), dim3(nwl), dim3(kGThreads), 0, s, T, wl, sorted, src, occ, off, ct, g,
-                           mis_count, mis_list);
+                           mis_count, mis_list, perm);
     else
         hipLaunchKernelGGL((kb_gather<0, V>), dim3(nwl), dim3(kGThreads), 0, s, T, wl, sorted, src, occ, off, ct, g,
-                           mis_count, mis_list);
+                           mis_count, mis_list, perm);
     return hipGetLastError();
 }
 }  // namespace
@@ -136,12 +138,16 @@ hipError_t launch_batch_gather_v(const BatchTank* T, const WorkItem* wl, uint32_
 hipError_t launch_batch_gather(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int brick_variant, int bf16,
                                const ParticleSoA& sorted, const ParticleSoA& src, const uint32_t* occ,
                                const uint32_t* cell_offset, const uint8_t* ct, const BatchGrid& g,
-                               const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s) {
+                               const uint32_t* mis_count, const uint32_t* mis_list, hipStream_t s,
+                               const uint32_t* perm) {
     if (nwl == 0) return hipSuccess;
     switch (brick_variant) {
-        case 0: return launch_batch_gather_v<0>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list, s);
-        case 1: return launch_batch_gather_v<1>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list, s);
-        case 2: return launch_batch_gather_v<2>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list, s);
+        case 0: return launch_batch_gather_v<0>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list,
+                                                    perm, s);
+        case 1: return launch_batch_gather_v<1>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list,
+                                                    perm, s);
+        case 2: return launch_batch_gather_v<2>(T, wl, nwl, bf16, sorted, src, occ, cell_offset, ct, g, mis_count, mis_list,
+                                                    perm, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -606,7 +612,8 @@ hipError_t launch_batch_gradient_enforce(const BatchTank* T, const WorkItem* wl,
 
 template <int BF16>
 __global__ __launch_bounds__(kBatchBlock) void kb_g2p(const BatchTank* __restrict__ T,
-                                                      const WorkItem* __restrict__ wl, ParticleSoA sp, BatchGrid g) {
+                                                      const WorkItem* __restrict__ wl, ParticleSoA sp, ParticleSoA rd,
+                                                      const uint32_t* __restrict__ perm, BatchGrid g) {
     const WorkItem wi = wl[blockIdx.x];
     if (wi.tank == kNoTank) return;
     const BatchTank& bt = T[wi.tank];
@@ -614,15 +621,18 @@ __global__ __launch_bounds__(kBatchBlock) void kb_g2p(const BatchTank* __restric
     const uint32_t local = wi.start + threadIdx.x;
     if (local >= P.N) return;
     const uint32_t* fo = bt.face_off;
-    g2p_particle<BF16>(P, sp, bt.part_off + local, g.vel[0] + fo[0], g.vel[1] + fo[1], g.vel[2] + fo[2],
-                       g.saved[0] + fo[0], g.saved[1] + fo[1], g.saved[2] + fo[2]);
+    const uint32_t idx = bt.part_off + local;
+    g2p_particle_from<BF16>(P, rd, perm ? perm[idx] : idx, sp, idx, g.vel[0] + fo[0], g.vel[1] + fo[1],
+                            g.vel[2] + fo[2], g.saved[0] + fo[0], g.saved[1] + fo[1], g.saved[2] + fo[2]);
 }
 
 hipError_t launch_batch_g2p(const BatchTank* T, const WorkItem* wl, uint32_t nwl, int bf16, const ParticleSoA& sp,
-                            const BatchGrid& g, hipStream_t s) {
+                            const BatchGrid& g, hipStream_t s, const ParticleSoA* from, const uint32_t* perm) {
     if (nwl == 0) return hipSuccess;
-    if (bf16) hipLaunchKernelGGL(kb_g2p<1>, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, g);
-    else hipLaunchKernelGGL(kb_g2p<0>, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, g);
+    if ((from == nullptr) != (perm == nullptr)) return hipErrorInvalidValue;
+    const ParticleSoA rd = from ? *from : sp;
+    if (bf16) hipLaunchKernelGGL(kb_g2p<1>, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, rd, perm, g);
+    else hipLaunchKernelGGL(kb_g2p<0>, dim3(nwl), dim3(kBatchBlock), 0, s, T, wl, sp, rd, perm, g);
     return hipGetLastError();
 }
 
