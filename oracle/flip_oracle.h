@@ -97,6 +97,13 @@ void orc_p2g_f32(const orc_params *P, uint32_t n, const float *px, const float *
 void orc_p2g_f64(const orc_params *P, uint32_t n, const double *px, const double *py,
                  const double *pz, const double *vx, const double *vy, const double *vz,
                  int64_t *num[3], int64_t *den[3]);
+/* normalize (sh/normalize.wgsl:15-27), save_vel (sh/save_vel.wgsl:14-19),
+ * forces (sh/forces.wgsl:30-84) and the pre-pressure enforce
+ * (sh/boundaries.wgsl:20-61), all three axes, from P2G's num/den. */
+void orc_p2g_epilogue_f32(const orc_params *P, const uint32_t *ct, int32_t *num[3], int32_t *den[3],
+                          float *vel[3], float *saved[3]);
+void orc_p2g_epilogue_f64(const orc_params *P, const uint32_t *ct, int64_t *num[3], int64_t *den[3],
+                          double *vel[3], double *saved[3]);
 
 /* ---- grid passes ---- */
 void orc_divergence_f32(const orc_params *P, float *const vel[3], const uint32_t *ct,

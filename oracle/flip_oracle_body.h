@@ -621,6 +621,11 @@ void FN(p2g)(const orc_params *P, uint32_t n, const real *px, const real *py, co
     p2g(P, n, px, py, pz, vx, vy, vz, (ACC **)num, (ACC **)den);
 }
 
+void FN(p2g_epilogue)(const orc_params *P, const uint32_t *ct, ACC_PUB *num[3], ACC_PUB *den[3], real *vel[3],
+                      real *saved[3]) {
+    p2g_epilogue(P, ct, (ACC **)num, (ACC **)den, vel, saved);
+}
+
 static void permute(uint32_t n, const uint32_t *perm, real *a, real *tmp) {
 #pragma omp parallel for schedule(static)
     for (int64_t d = 0; d < (int64_t)n; ++d) tmp[d] = a[perm[d]];

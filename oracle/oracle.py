@@ -182,6 +182,22 @@ def p2g(P, pos, vel):
     return num, den
 
 
+def p2g_epilogue(P, ct, num, den, f64=False):
+    """orc_p2g_epilogue_*: (vel, saved) per axis from P2G's num/den and the
+    cell types (normalize, save, forces, pre-pressure enforce)."""
+    real = np.float64 if f64 else np.float32
+    acc = np.int64 if f64 else np.int32
+    ct = np.ascontiguousarray(ct, dtype=np.uint32)
+    num = [np.array(x, dtype=acc, copy=True) for x in num]
+    den = [np.array(x, dtype=acc, copy=True) for x in den]
+    vel = [np.zeros(face_count(P, a), real) for a in range(3)]
+    saved = [np.zeros(face_count(P, a), real) for a in range(3)]
+    arr = lambda xs: (C.c_void_p * 3)(*[ptr(x) for x in xs])
+    fn = lib().orc_p2g_epilogue_f64 if f64 else lib().orc_p2g_epilogue_f32
+    fn(C.byref(P), ptr(ct), arr(num), arr(den), arr(vel), arr(saved))
+    return vel, saved
+
+
 def divergence(P, vel, ct, occ=None):
     vel = [np.ascontiguousarray(v, dtype=np.float32) for v in vel]
     ct = np.ascontiguousarray(ct, dtype=np.uint32)

@@ -270,3 +270,134 @@ def test_full_step_conserves_particles_and_keeps_them_inside(orc):
     assert r.pos.min() >= np.float32(1.05 * h) - 1e-7 and r.pos.max() <= 1 - 1.05 * h + 1e-7
     assert int(r.occupancy.sum()) == len(sc["pos"])
     assert np.all(np.isfinite(r.vel_particles))
+
+
+# ---------------------------------------------------------------------------
+# Face epilogue: normalize (sh/normalize.wgsl:15-27), save (sh/save_vel.wgsl:
+# 14-19), forces (sh/forces.wgsl:30-84) and the pre-pressure enforce
+# (sh/boundaries.wgsl:20-61), checked face by face against a brute-force
+# statement built from the face's two neighbouring cells.
+
+EPI_DIMS = (6, 7, 5)  # non-cubic: a transposed index or a swapped axis shows
+
+
+def _epilogue_case(seed, dims=EPI_DIMS, interior_solids=3):
+    """Cell types with a solid shell, random liquid / air inside and a few
+    interior solid cells; num/den with zero, negative (wrapped) and
+    positive denominators, num nonzero where den is zero."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    ct3 = np.zeros((nz, ny, nx), np.uint32)  # [k, j, i]
+    ct3[1:-1, 1:-1, 1:-1] = rng.choice(np.array([1, 2], np.uint32), size=(nz - 2, ny - 2, nx - 2), p=[0.55, 0.45])
+    inner = np.argwhere(ct3[1:-1, 1:-1, 1:-1] >= 0) + 1
+    for k, j, i in inner[rng.choice(len(inner), interior_solids, replace=False)]:
+        ct3[k, j, i] = 0
+    num, den = [], []
+    for a in range(3):
+        d = [nx, ny, nz]
+        d[a] += 1
+        cnt = d[0] * d[1] * d[2]
+        dd = rng.integers(1, 200_000, cnt).astype(np.int64)
+        kind = rng.random(cnt)
+        dd[kind < 0.25] = 0
+        dd[(kind >= 0.25) & (kind < 0.3)] = -rng.integers(1, 1000, int(np.sum((kind >= 0.25) & (kind < 0.3))))
+        nn = rng.integers(-3_000_000, 3_000_000, cnt).astype(np.int64)
+        num.append(nn.astype(np.int32))
+        den.append(dd.astype(np.int32))
+    return ct3, num, den
+
+
+def _epilogue_expected(dims, ct3, num, den, grav, dt, real=np.float32, mutate=None):
+    """Per axis a, on the face array [k, j, i] (n_a + 1 along a): the lo cell
+    is the face's coordinate minus one and the hi cell its coordinate along
+    a, a cell outside the grid counting as solid.  saved = num/den where
+    den > 0 else 0; vel = saved + g_a*dt where the lo or hi cell is liquid;
+    then vel = 0 where the lo or hi cell is solid.  `mutate` plants one
+    plausible mistake in this statement (test of the test's reach)."""
+    nx, ny, nz = dims
+    outs_v, outs_s = [], []
+    for a in range(3):
+        d = [nx, ny, nz]
+        d[a] += 1
+        shape = (d[2], d[1], d[0])
+        n3 = num[a].reshape(shape).astype(np.int64)
+        d3 = den[a].reshape(shape).astype(np.int64)
+        ax = 2 - a  # numpy axis of the face's own direction
+        keep = d3 > 0 if mutate != "den_nonzero" else d3 != 0
+        safe = np.where(keep, d3, 1)
+        saved = np.where(keep, real(1) * n3.astype(real) / safe.astype(real), real(0)).astype(real)
+        if mutate == "den_zero_keeps_num":
+            saved = np.where(d3 == 0, n3.astype(real), saved).astype(real)
+        pad_w = [(0, 0)] * 3
+        pad_w[ax] = (1, 1)
+        ctp = np.pad(ct3, pad_w, constant_values=0)  # outside the grid: solid
+        lo = np.take(ctp, np.arange(0, shape[ax]), axis=ax)
+        hi = np.take(ctp, np.arange(1, shape[ax] + 1), axis=ax)
+        if mutate == "hi_cell_shifted":
+            hi = np.take(np.pad(ctp, pad_w, constant_values=0), np.arange(3, shape[ax] + 3), axis=ax)
+        wet = (lo == 1) | (hi == 1)
+        if mutate == "gravity_on_dry_faces":
+            wet = np.ones_like(wet)
+        g_a = grav[(a + 1) % 3] if mutate == "gravity_axis_rotated" else grav[a]
+        gdt = real(g_a) * real(dt)
+        vel = np.where(wet, saved + gdt, saved).astype(real)
+        solid = (lo == 0) | (hi == 0)
+        if mutate == "interior_solids_ignored":
+            coord = np.indices(shape)[ax]
+            solid = (coord == 0) | (coord == shape[ax] - 1)
+        vel = np.where(solid, real(0), vel).astype(real)
+        outs_v.append(vel.reshape(-1))
+        outs_s.append(saved.reshape(-1))
+    return outs_v, outs_s
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("f64", [False, True])
+def test_face_epilogue_matches_the_per_face_closed_form(orc, seed, f64):
+    nx, ny, nz = EPI_DIMS
+    grav, dt = (0.7, -9.81, 2.3), 1.0 / 120.0
+    P = orc.make_params(nx, ny, nz, 0.125, dt=dt, grav=grav)
+    ct3, num, den = _epilogue_case(seed)
+    vel, saved = orc.p2g_epilogue(P, ct3.reshape(-1), num, den, f64=f64)
+    real = np.float64 if f64 else np.float32
+    ev, es = _epilogue_expected(EPI_DIMS, ct3, num, den, [real(np.float32(g)) for g in grav],
+                                real(np.float32(dt)), real=real)
+    for a in range(3):
+        assert np.array_equal(saved[a], es[a]), f"saved axis {a}"
+        assert np.array_equal(vel[a], ev[a]), f"vel axis {a}"
+    # the case exercises every branch: empty, negative and positive den;
+    # wet and dry faces; faces zeroed by interior solids
+    assert any(np.any(d == 0) and np.any(d < 0) for d in den)
+    assert any(np.any((v != 0) & (s == 0)) for v, s in zip(vel, saved))  # gravity on a wet empty face
+
+
+@pytest.mark.parametrize("mutation", ["gravity_on_dry_faces", "den_nonzero", "den_zero_keeps_num",
+                                      "hi_cell_shifted", "interior_solids_ignored", "gravity_axis_rotated"])
+def test_face_epilogue_closed_form_rejects_planted_mistakes(orc, mutation):
+    """Each plausible mistake, planted in the closed form, disagrees with the
+    restatement on this case: the per-face check above would catch it."""
+    nx, ny, nz = EPI_DIMS
+    grav, dt = (0.7, -9.81, 2.3), 1.0 / 120.0
+    P = orc.make_params(nx, ny, nz, 0.125, dt=dt, grav=grav)
+    ct3, num, den = _epilogue_case(0)
+    vel, saved = orc.p2g_epilogue(P, ct3.reshape(-1), num, den)
+    g32 = [np.float32(g) for g in grav]
+    ev, es = _epilogue_expected(EPI_DIMS, ct3, num, den, g32, np.float32(dt), mutate=mutation)
+    with np.errstate(invalid="ignore"):
+        same = all(np.array_equal(vel[a], ev[a]) and np.array_equal(saved[a], es[a]) for a in range(3))
+    assert not same
+
+
+def test_face_epilogue_of_the_step_matches_the_closed_form(orc):
+    """Inside a full substep: the restatement's vel/saved after P2G equal the
+    closed form applied to its own num/den and cell types."""
+    import scenes
+    from helpers import oracle_params
+    sc = scenes.small_tank(n=12, seed=8, fill=0.6)
+    P = oracle_params(sc, 0)
+    r = orc.run(P, sc["pos"], sc["vel"], steps=1)
+    ct3 = r.cell_type.reshape(12, 12, 12)
+    ev, es = _epilogue_expected((12, 12, 12), ct3, r.num, r.den, [np.float32(g) for g in P.grav],
+                                np.float32(P.dt))
+    for a in range(3):
+        assert np.array_equal(r.saved[a], es[a]) and np.array_equal(r.vel[a], ev[a])
