@@ -44,7 +44,8 @@ struct fs_batch {
     uint32_t* mis_list = nullptr;
     BatchCgWork cg{};
     int cg_max_blocks = 0;      // co-resident figure of the CG kernel
-    int cg_default_blocks = 0;  // half of it, whole XCD groups
+    int cg_default_blocks = 0;  // persistent_grid_blocks of it (or a smaller override)
+    int cg_limit_blocks = 0;    // the largest grid an override may ask for
     int cg_grid = 0;            // workgroups of the current CG plan (padded to 8)
 
     fs_batch_tuning tuning{};
@@ -409,10 +410,13 @@ FS_API int fs_batch_create(const fs_params* params, const float* const* pos_xyz,
         int dev = 0, cus = 0;
         FS_HIP(hipGetDevice(&dev));
         FS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // Overrides may only shrink the grid below the default sizing: a grid
+        // above it is not launched (see persistent_grid_blocks).
         bt->cg_default_blocks = persistent_grid_blocks(bt->cg_max_blocks, cus);
+        bt->cg_limit_blocks = bt->cg_default_blocks;
         if (const char* env = std::getenv("FLUIDSTEP_CG_BLOCKS")) {
             const int v = std::atoi(env);
-            if (v > 0 && v <= bt->cg_max_blocks) bt->cg_default_blocks = v;
+            if (v > 0 && v <= bt->cg_limit_blocks) bt->cg_default_blocks = v;
         }
         FS_HIP(bt->alloc(bt->cg.partials, 2 * ((size_t)bt->cg_max_blocks + 8)));
         FS_HIP(bt->alloc(bt->cg.flags, 2 * ((size_t)bt->cg_max_blocks + 8)));
@@ -513,6 +517,19 @@ FS_API int fs_batch_step(fs_batch* bt, uint32_t substeps) {
 FS_API int fs_batch_synchronize(fs_batch* bt) {
     if (!bt) return fail(FS_ERR_INVALID_ARGUMENT, "null batch handle");
     FS_HIP(hipStreamSynchronize(bt->stream));
+    // A solve that timed out in any substep since the last check leaves its
+    // tank's sticky word set; report it once and clear it.
+    std::vector<BatchCgState> st(bt->T);
+    FS_HIP(hipMemcpy(st.data(), bt->cg.state, st.size() * sizeof(BatchCgState), hipMemcpyDeviceToHost));
+    int first = -1;
+    for (uint32_t t = 0; t < bt->T; ++t)
+        if (st[t].failed) {
+            if (first < 0) first = (int)t;
+            const uint32_t zero = 0u;
+            FS_HIP(hipMemcpy(&bt->cg.state[t].failed, &zero, sizeof zero, hipMemcpyHostToDevice));
+        }
+    if (first >= 0)
+        return fail(FS_ERR_SOLVER, "batched CG group barrier timed out (tank " + std::to_string(first) + ")");
     return FS_OK;
 }
 
@@ -527,7 +544,9 @@ FS_API int fs_batch_set_tuning(fs_batch* bt, const fs_batch_tuning* tuning) {
     if (int rc = check_tuning(*tuning)) return rc;
     FS_HIP(hipStreamSynchronize(bt->stream));
     int G = tuning->cg_blocks ? (int)tuning->cg_blocks : bt->cg_default_blocks;
-    if (G > bt->cg_max_blocks) return fail(FS_ERR_INVALID_ARGUMENT, "cg_blocks above the co-resident figure");
+    if (G > bt->cg_limit_blocks)
+        return fail(FS_ERR_INVALID_ARGUMENT, "cg_blocks above the persistent grid limit (" +
+                                                 std::to_string(bt->cg_limit_blocks) + ")");
     if (int rc = plan_cg(bt, G)) return rc;
     bt->tuning = *tuning;
     bt->tuning.cg_blocks = (uint32_t)G;
@@ -634,7 +653,8 @@ FS_API int fs_batch_autotune(fs_batch* bt, fs_batch_tuning* out) {
             std::vector<BatchCgState> st(bt->T);
             FS_HIP(hipMemcpy(st.data(), bt->cg.state, st.size() * sizeof(BatchCgState), hipMemcpyDeviceToHost));
             for (const auto& x : st)
-                if (x.error) return fail(FS_ERR_SOLVER, "batched CG group barrier timed out during tuning");
+                if (x.error || x.failed)
+                    return fail(FS_ERR_SOLVER, "batched CG group barrier timed out during tuning");
             if (ms < best_ms) {
                 best_ms = ms;
                 best.cg_unroll = (uint32_t)u;
@@ -755,7 +775,7 @@ FS_API int fs_batch_get_stats(fs_batch* bt, uint32_t tank, fs_stats* out) {
     FS_HIP(hipMemcpy(&ts[1], bt->cg.tile_scan + k.tile_off + k.ntiles, 4, hipMemcpyDeviceToHost));
     BatchCgState st;
     FS_HIP(hipMemcpy(&st, bt->cg.state + tank, sizeof st, hipMemcpyDeviceToHost));
-    if (st.error) return fail(FS_ERR_SOLVER, "batched CG group barrier timed out");
+    if (st.error || st.failed) return fail(FS_ERR_SOLVER, "batched CG group barrier timed out");
     s.liquid_cells = cnt[0];
     s.misfits = cnt[1];
     s.sorted = bt->substeps ? 1u : 0u;

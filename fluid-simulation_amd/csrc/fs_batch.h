@@ -34,7 +34,10 @@ constexpr int kBatchBlock = 256;  // particles, cells or faces per workgroup
 struct BatchCgState {
     double rs_init, rs_final, true_res_sq;
     uint32_t iterations;
-    uint32_t error;  // nonzero: the group's barrier timed out
+    uint32_t error;   // nonzero: the group's barrier timed out in the last solve
+    uint32_t failed;  // sticky: a solve of this tank timed out since the host
+                      // last cleared it (kernels only ever set it)
+    uint32_t pad;
 };
 
 struct BatchCgWork {

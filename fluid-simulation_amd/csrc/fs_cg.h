@@ -122,9 +122,9 @@ __device__ __forceinline__ void publish_tagged(uint64_t* half, uint32_t slot, ui
 }
 
 // Waits for slot `slot` to carry `epoch`; false (and the abort flag raised)
-// on abort or after kSpinTimeoutTicks.
+// on abort or after `limit` wall_clock64 ticks.
 __device__ __forceinline__ bool poll_tagged(const uint64_t* half, uint32_t slot, uint32_t epoch, uint32_t* abort_flag,
-                                            double& v, uint32_t& flag) {
+                                            double& v, uint32_t& flag, long long limit = kSpinTimeoutTicks) {
     const long long t0 = wall_clock64();
     for (uint32_t spin = 0;; ++spin) {
         const uint64_t hi = __hip_atomic_load(half + 2u * slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -134,7 +134,7 @@ __device__ __forceinline__ bool poll_tagged(const uint64_t* half, uint32_t slot,
             flag = (uint32_t)(lo >> 32) & 1u;
             return true;
         }
-        if ((spin & 31u) == 31u && (ld_relaxed(abort_flag) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+        if ((spin & 31u) == 31u && (ld_relaxed(abort_flag) || wall_clock64() - t0 > limit)) {
             __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return false;
         }

@@ -307,19 +307,27 @@ bool env_is(const char* name, const char* value) {
     return e && std::strcmp(e, value) == 0;
 }
 
+// The sorted particles are now on the other side: it becomes the live one.
+void flip_live_side(fs_tank* t) {
+    t->cur ^= 1;
+    t->sort_swapped = true;  // src/gpu/fluid.rs:1570
+    t->perm_valid = 1;
+}
+
 // Stable sort of the live side into the other side (both engines).  With
 // move = false the permutation is computed but not applied: the particles
-// stay on what becomes the other side, and the caller fills the new live
-// side through t->perm.
+// stay where they are (t->cur unchanged), and the caller fills the other
+// side through t->perm and only then calls flip_live_side, so an error
+// return in between leaves the tank on its complete, unsorted side.
 int sort_particles(fs_tank* t, bool move = true) {
     const StepParams& P = t->P;
     hipStream_t s = t->stream;
     FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
     FS_HIP(launch_stable_permutation(t->keys, P.N, P.C, t->ktmp0, t->ktmp1, t->vtmp, t->perm, t->radix_ws, s));
-    if (move) FS_HIP(launch_permute(P.N, t->perm, t->side[t->cur], t->side[t->cur ^ 1], t->bf16, s));
-    t->cur ^= 1;
-    t->sort_swapped = true;  // src/gpu/fluid.rs:1570
-    t->perm_valid = 1;
+    if (move) {
+        FS_HIP(launch_permute(P.N, t->perm, t->side[t->cur], t->side[t->cur ^ 1], t->bf16, s));
+        flip_live_side(t);
+    }
     return FS_OK;
 }
 
@@ -432,13 +440,16 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     FS_MARK(t, kSecMark);
     if (sort_now) {
         // The permutation is applied by G2P, which rewrites every particle
-        // anyway; until then the particles lie on the pre-sort side.
+        // anyway; until then the particles lie on the pre-sort side, which
+        // stays the live side until G2P is enqueued.
         int rc = sort_particles(t, false);
         if (rc != FS_OK) return rc;
     }
     FS_MARK(t, kSecSort);
-    ParticleSoA live = t->side[t->cur];
-    const ParticleSoA pre = t->side[t->cur ^ 1];
+    // live: the side G2P writes (the sorted side on a sort substep);
+    // pre: the side the particles lie on now.
+    ParticleSoA live = t->side[sort_now ? t->cur ^ 1 : t->cur];
+    const ParticleSoA pre = t->side[t->cur];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
     FS_MARK(t, kSecClassify);
     if (sort_now || t->gperm) {
@@ -450,12 +461,17 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
         // form's terms in any order.
         const ParticleSoA& from = sort_now ? pre : live;
         const uint32_t* order = t->perm;
+        // Bin starts of this order: the sort's (cell_offset) or, without a
+        // sort, the cell order's own (goff), so that cell_offset keeps the
+        // bin starts of the last sort as the reference's buffer does.
+        const uint32_t* starts = t->cell_offset;
         if (!sort_now) {
-            FS_HIP(launch_exclusive_scan(t->occ, t->cell_offset, P.C, t->scan_ws, s));
-            FS_HIP(launch_cell_order(t->keys, P.N, t->cell_offset, t->cursor, P.C, t->gperm, s));
+            FS_HIP(launch_exclusive_scan(t->occ, t->goff, P.C, t->scan_ws, s));
+            FS_HIP(launch_cell_order(t->keys, P.N, t->goff, t->cursor, P.C, t->gperm, s));
             order = t->gperm;
+            starts = t->goff;
         }
-        FS_HIP(launch_p2g_gather(P, from, from, t->bf16, t->occ, t->cell_offset, t->ct, t->vel, t->saved,
+        FS_HIP(launch_p2g_gather(P, from, from, t->bf16, t->occ, starts, t->ct, t->vel, t->saved,
                                  t->counters + 1, t->mis_list, s, t->gather_brick, order));
         // Fallback when more particles' P2G cells differ from their sort
         // cells than the gather walks (any particle order will do).
@@ -514,13 +530,18 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     FS_HIP(launch_gradient_enforce(P, t->p, t->ct, t->vel, s));
     FS_MARK(t, kSecGradient);
     if (stop_before_g2p) {
-        if (sort_now) FS_HIP(launch_permute(P.N, t->perm, pre, live, t->bf16, s));
+        if (sort_now) {
+            FS_HIP(launch_permute(P.N, t->perm, pre, live, t->bf16, s));
+            flip_live_side(t);
+        }
         return FS_OK;
     }
-    if (sort_now)
+    if (sort_now) {
         FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s, &pre, t->perm));
-    else
+        flip_live_side(t);
+    } else {
         FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
+    }
     FS_MARK(t, kSecG2p);
     if (t->timing) FS_HIP(hipEventRecord(t->ev[3], s));
     return FS_OK;
@@ -597,6 +618,12 @@ namespace {
 int check_tank(const fs_tank* t) {
     if (!t) return fail(FS_ERR_INVALID_ARGUMENT, "null tank handle");
     return FS_OK;
+}
+
+// Clears the solver's sticky failure word (after it was reported, or on reset).
+hipError_t clear_solver_failure(fs_tank* t) {
+    const uint32_t zero = 0u;
+    return hipMemcpy(&t->cg.state->failed, &zero, sizeof zero, hipMemcpyHostToDevice);
 }
 
 }  // namespace
@@ -716,6 +743,7 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         const bool unsorted_substeps = !q.particle_sort || q.particle_sort_period > 1;
         if (q.engine == FS_ENGINE_FUSED && unsorted_substeps && !env_is("FLUIDSTEP_UNSORTED_P2G", "atomic")) {
             FS_HIP(t->alloc(t->gperm, n));
+            FS_HIP(t->alloc(t->goff, C));
             FS_HIP(t->alloc(t->cursor, C));
         }
         FS_HIP(t->alloc(t->radix_ws, radix_workspace(n)));
@@ -753,9 +781,10 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         if (t->cg_blocks <= 0) return fail(FS_ERR_DEVICE, "cannot size the persistent CG grid");
         const int max_blocks = t->cg_blocks;
         t->cg_blocks = persistent_grid_blocks(max_blocks, cus);
+        // The override may only shrink the grid below the default sizing.
         if (const char* env = std::getenv("FLUIDSTEP_CG_BLOCKS")) {
             int v = std::atoi(env);
-            if (v > 0 && v <= max_blocks) t->cg_blocks = v;
+            if (v > 0 && v <= t->cg_blocks) t->cg_blocks = v;
         }
         if (t->cg_blocks <= 0) t->cg_blocks = 1;
         t->cg_multi_blocks = 4 * cus;
@@ -840,6 +869,7 @@ FS_API int fs_reset(fs_tank* t) {
     }
     FS_HIP(launch_fill_f32(t->p, 0.0f, t->P.C, t->stream));
     FS_HIP(hipStreamSynchronize(t->stream));
+    FS_HIP(clear_solver_failure(t));
     t->tick = 0;
     t->substeps = 0;
     t->perm_valid = 0;
@@ -921,10 +951,16 @@ FS_API int fs_pack_cell_types_u32(fs_tank* t, uint32_t* dst_device) {
 FS_API int fs_synchronize(fs_tank* t) {
     if (int rc = check_tank(t)) return rc;
     FS_HIP(hipStreamSynchronize(t->stream));
-    if (t->prm.engine == FS_ENGINE_FUSED && t->substeps > 0) {
+    if (t->prm.engine == FS_ENGINE_FUSED) {
+        // A solve that timed out in any substep since the last check leaves
+        // the sticky word set (a later successful solve does not clear it):
+        // report it once and clear it.
         CgState st;
         FS_HIP(hipMemcpy(&st, t->cg.state, sizeof st, hipMemcpyDeviceToHost));
-        if (st.error) return fail(FS_ERR_SOLVER, "persistent CG grid barrier timed out");
+        if (st.failed) {
+            FS_HIP(clear_solver_failure(t));
+            return fail(FS_ERR_SOLVER, "persistent CG grid barrier timed out");
+        }
     }
     return FS_OK;
 }
@@ -1081,7 +1117,7 @@ FS_API int fs_get_stats(fs_tank* t, fs_stats* out) {
     if (t->prm.engine == FS_ENGINE_FUSED) {
         CgState st;
         FS_HIP(hipMemcpy(&st, t->cg.state, sizeof st, hipMemcpyDeviceToHost));
-        if (st.error) return fail(FS_ERR_SOLVER, "persistent CG grid barrier timed out");
+        if (st.error || st.failed) return fail(FS_ERR_SOLVER, "persistent CG grid barrier timed out");
         s.cg_iterations = st.iterations;
         s.rs_initial = st.rs_init;
         s.rs_final = st.rs_old;

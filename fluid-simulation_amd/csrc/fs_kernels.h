@@ -126,8 +126,9 @@ struct CgState {
     float alpha, beta;       // f32 step lengths of the current iteration
     uint32_t iterations;
     uint32_t active;
-    uint32_t error;          // nonzero: a grid barrier timed out
-    uint32_t pad;
+    uint32_t error;          // nonzero: the last solve's grid barrier timed out
+    uint32_t failed;         // sticky: some solve timed out since the host last
+                             // cleared it (kernels only ever set it)
 };
 struct CgWork {
     // grid-barrier + reduction scratch; every word below up to `slots_data`
@@ -149,6 +150,7 @@ struct CgWork {
     uint32_t* tiles;         // [C/64] active tile list
     uint32_t* n_tiles;       // [1]
     CgState* state;          // [1]
+    long long spin_ticks;    // limit of every bounded wait (wall_clock64 ticks)
 };
 // tiles[scan[t]] = t for flagged tiles; *n_active = the flagged count.
 hipError_t launch_compact_tiles(uint32_t ntiles, const uint32_t* flags, const uint32_t* scan, uint32_t* tiles,

@@ -33,6 +33,7 @@ struct fs_tank {
     // Fused engine, substeps without a sort: the live side's stable cell order,
     // through which the gather reads the particles (they stay in place).
     uint32_t* gperm = nullptr;
+    uint32_t* goff = nullptr;    // bin starts of gperm
     uint32_t* cursor = nullptr;  // per-cell slot counters while gperm is built
     uint32_t *radix_ws = nullptr, *scan_ws = nullptr;
     uint32_t *occ = nullptr, *cell_offset = nullptr;

@@ -457,6 +457,7 @@ __global__ __launch_bounds__(kBCgThreads) void kb_cg(const BatchTank* __restrict
             st->true_res_sq = tr;
             st->iterations = iters;
             st->error = err;
+            if (err) st->failed = 1u;
         }
     };
 

@@ -35,6 +35,12 @@
 #include "fs_cg.h"
 
 namespace fs {
+// A solve whose grid barrier timed out: the last-solve word and the sticky
+// word (cleared only by the host) are both set.
+__device__ inline void cg_mark_failed(CgState* st) {
+    st->error = 1u;
+    st->failed = 1u;
+}
 
 namespace {
 inline unsigned grid_for(size_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
@@ -56,6 +62,7 @@ void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks) {
     w.bar_gen = area + kSyncGen;
     w.slots = reinterpret_cast<uint64_t*>(area + kSyncSlots);
     w.slots_data = reinterpret_cast<uint64_t*>(area + kSyncSlots + 8u * (size_t)(max_blocks > 0 ? max_blocks : 1));
+    w.spin_ticks = kSpinTimeoutTicks;
 }
 
 hipError_t launch_cg_reset(const CgWork& w, hipStream_t s) { return launch_fill_u32(w.sync, 0u, w.sync_words, s); }
@@ -170,7 +177,7 @@ __device__ bool grid_sum_nv(const double (&v)[NV], const CgWork& w, uint32_t G, 
                 s_abort = 1;
                 break;
             }
-            if (wall_clock64() - t0 > kSpinTimeoutTicks) {
+            if (wall_clock64() - t0 > w.spin_ticks) {
                 __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 s_abort = 1;
                 break;
@@ -228,7 +235,7 @@ __device__ __forceinline__ bool grid_exchange_all(double v, const CgWork& w, uin
     for (uint32_t i = threadIdx.x; i < G; i += THREADS) {
         double x;
         uint32_t f;
-        if (!poll_tagged(half, i, epoch, w.abort_flag, x, f)) {
+        if (!poll_tagged(half, i, epoch, w.abort_flag, x, f, w.spin_ticks)) {
             s_abort = 1;
             break;
         }
@@ -304,7 +311,7 @@ __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* 
                 }
             }
             if (pending && (spin & 31u) == 31u &&
-                (ld_relaxed(w.abort_flag) || wall_clock64() - t0 > kSpinTimeoutTicks)) {
+                (ld_relaxed(w.abort_flag) || wall_clock64() - t0 > w.spin_ticks)) {
                 __hip_atomic_store(w.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = false;
                 break;
@@ -549,7 +556,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
         ok = cg_two_phase<THREADS, EXCH, DB, false, false>(P, b, p, r, d0, d1, q, w, G, tw, tile_id, rs_init,
                                                            rs_old, iters);
     if (!ok) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mark_failed(w.state);
         return;
     }
     // Resident blocks store p after the loop and streaming ones may finish a
@@ -559,7 +566,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     const bool okp = use_tagged<DB>(G) ? grid_sum_tagged<THREADS>(0.0, w, G, iters + 2u, unused)
                                        : grid_sum<THREADS>(0.0, w, G, unused);
     if (!okp) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mark_failed(w.state);
         return;
     }
     double acc = 0.0;
@@ -567,7 +574,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_persistent(StepParams P, const u
     double tr;
     const bool ok2 = EXCH ? grid_exchange<THREADS>(acc, w, G, iters + 1u, tr) : grid_sum<THREADS>(acc, w, G, tr);
     if (!ok2) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mark_failed(w.state);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -613,7 +620,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
         cgs_init_batch<U>(P, w.nmask, v.r[0], v.w[0], tw, ti, tile_id, part[0], part[1]);
     if (!grid_sum_nv<THREADS, 2>(part, w, G, tot)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mark_failed(w.state);
         return;
     }
     const double rs_init = tot[0];
@@ -641,7 +648,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v
         for (uint32_t ti = tw.begin; ti < tw.end; ti += step)
             cgs_sweep_batch<U>(P, w.nmask, io, alpha, beta, first, tw, ti, tile_id, part[0], part[1]);
         if (!grid_sum_nv<THREADS, 2>(part, w, G, tot)) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+            if (blockIdx.x == 0 && threadIdx.x == 0) cg_mark_failed(w.state);
             return;
         }
         gam_prev = gam;
@@ -654,7 +661,7 @@ __global__ __launch_bounds__(THREADS) void k_cg_single(StepParams P, CgVectors v
     for (uint32_t ti = tw.begin; ti < tw.end; ti += step) acc += true_res_batch<U>(P, w.nmask, v.b, v.x, tw, ti, tile_id);
     double tr;
     if (!grid_exchange<THREADS>(acc, w, G, 1u, tr)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) w.state->error = 1u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) cg_mark_failed(w.state);
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -731,9 +738,17 @@ int cg_persistent_max_blocks() {
     return cus * per_cu;
 }
 
-hipError_t launch_cg_persistent(const StepParams& P, const CgVectors& v, const CgWork& w, int grid_blocks,
+hipError_t launch_cg_persistent(const StepParams& P, const CgVectors& v, const CgWork& w_in, int grid_blocks,
                                 hipStream_t s) {
     const PersistentVariant pv = persistent_variant();
+    // FLUIDSTEP_CG_SPIN_TICKS shortens the bounded waits (read at every
+    // launch): a test forces the solver-failure path with it, every block
+    // still leaving the kernel through the abort flag.
+    CgWork w = w_in;
+    if (const char* env = std::getenv("FLUIDSTEP_CG_SPIN_TICKS")) {
+        const long long v_ = std::atoll(env);
+        if (v_ > 0 && v_ < w.spin_ticks) w.spin_ticks = v_;
+    }
     if (pv.single) {
         void* args[] = {(void*)&P, (void*)&v, (void*)&w};
         ++g_dispatches;
