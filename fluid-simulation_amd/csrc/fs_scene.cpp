@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "fluidstep.h"
@@ -258,6 +260,63 @@ FS_API int fs_scene_seed(const fs_scene* scene, float h, const float origin[3], 
         hipStreamSynchronize(s) != hipSuccess)
         return done(fail(FS_ERR_DEVICE, "seed copy failed"));
     return done(FS_OK);
+}
+
+// ---- fs_scene_seed_noalloc: the lattice staged in a fixed device array ----
+// The staging array of fs_scene_seed_noalloc: one per GPU, for the life of
+// the process.  Host code never passes `scratch` itself to a kernel or a copy
+// (on the host it names a stand-in, not device memory); it uses the device
+// address that hipGetSymbolAddress returns on each device.
+static __device__ __align__(256) float scratch[FS_SEED_SCRATCH_BYTES / sizeof(float)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device (with it current) and kept.
+int scratch_on_current_device(float** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(FS_ERR_DEVICE, "hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices) return fail(FS_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(FS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return FS_OK;
+}
+}  // namespace
+
+FS_API int fs_scene_seed_noalloc(const fs_scene* scene, float h, const float origin[3], float* pos_xyz,
+                                 uint32_t capacity, uint32_t* count) {
+    if (!scene || !origin || !count) return fail(FS_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(h > 0.0f) || !std::isfinite(h)) return fail(FS_ERR_INVALID_ARGUMENT, "h must be positive and finite");
+    SeedLayout L;
+    if (int rc = build_layout(*scene, h, origin, L); rc != FS_OK) return rc;
+    *count = L.total;
+    if (!pos_xyz) return FS_OK;
+    if (capacity < L.total) return fail(FS_ERR_INVALID_ARGUMENT, "pos_xyz holds fewer triplets than the lattice");
+    const size_t bytes = (size_t)L.total * 3u * sizeof(float);
+    if (bytes > FS_SEED_SCRATCH_BYTES)
+        return fail(FS_ERR_INVALID_ARGUMENT, "lattice of " + std::to_string(L.total) +
+                                                 " particles exceeds the staging array (" +
+                                                 std::to_string(FS_SEED_SCRATCH_PARTICLES) + " particles)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FS_ERR_DEVICE, "no HIP device available");
+    float* stage = nullptr;
+    if (int rc = scratch_on_current_device(&stage)) return rc;
+    // The kernel writes every staged float it covers, so nothing needs clearing.
+    if (launch_seed_lattice(L, stage, nullptr) != hipSuccess) return fail(FS_ERR_DEVICE, "seed kernel launch failed");
+    if (hipMemcpyAsync(pos_xyz, stage, bytes, hipMemcpyDeviceToHost, nullptr) != hipSuccess ||
+        hipStreamSynchronize(nullptr) != hipSuccess)
+        return fail(FS_ERR_DEVICE, "seed copy failed");
+    return FS_OK;
 }
 
 FS_API int fs_create_from_settings(const fs_registry* r, const fs_scene* scene, fs_tank** out) {

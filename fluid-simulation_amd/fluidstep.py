@@ -43,7 +43,8 @@ EXPORTED = (
     "fs_batch_buffer_bytes", "fs_batch_read_particles", "fs_batch_read_faces",
     "fs_batch_read_cells_f32", "fs_batch_read_cells_u32", "fs_batch_read_permutation",
     "fs_batch_get_stats", "fs_forward_checkpointed", "fs_step_adjoint",
-    "fs_scene_preset_blocks", "fs_scene_from_registry", "fs_scene_seed", "fs_create_from_settings",
+    "fs_scene_preset_blocks", "fs_scene_from_registry", "fs_scene_seed", "fs_scene_seed_noalloc",
+    "fs_create_from_settings",
     "fs_get_device_view", "fs_take_sort_swapped", "fs_pack_particles_vec4", "fs_pack_cell_types_u32",
     "fs_timestep_create", "fs_timestep_destroy", "fs_timestep_steps_for_frame", "fs_timestep_record_manual",
     "fs_timestep_record_export", "fs_timestep_record_idle", "fs_timestep_reset", "fs_timestep_last_stats",
@@ -326,6 +327,7 @@ def lib():
         L.fs_scene_from_registry.argtypes = [vp, C.POINTER(Scene)]
         L.fs_scene_seed.argtypes = [C.POINTER(Scene), C.c_float, C.POINTER(C.c_float), vp, C.c_uint32,
                                     C.POINTER(C.c_uint32)]
+        L.fs_scene_seed_noalloc.argtypes = L.fs_scene_seed.argtypes
         L.fs_create_from_settings.argtypes = [vp, C.POINTER(Scene), C.POINTER(vp)]
         _lib = L
     return _lib
@@ -546,6 +548,17 @@ def scene_seed(scene: Scene, h: float, origin, want: bool = True):
         return k.value
     out = np.zeros((k.value, 3), np.float32)
     _check(lib().fs_scene_seed(C.byref(scene), h, o, _ptr(out), k.value, C.byref(k)))
+    return out
+
+
+def scene_seed_noalloc(scene: Scene, h: float, origin):
+    """scene_seed through fs_scene_seed_noalloc (no device allocation: the
+    lattice is staged in the library's fixed per-GPU array)."""
+    o = (C.c_float * 3)(*origin)
+    k = C.c_uint32()
+    _check(lib().fs_scene_seed_noalloc(C.byref(scene), h, o, None, 0, C.byref(k)))
+    out = np.zeros((k.value, 3), np.float32)
+    _check(lib().fs_scene_seed_noalloc(C.byref(scene), h, o, _ptr(out), k.value, C.byref(k)))
     return out
 
 

@@ -456,6 +456,19 @@ FS_API int fs_scene_from_registry(const fs_registry* r, fs_scene* out);
  * reference's sequential generator. */
 FS_API int fs_scene_seed(const fs_scene* scene, float h, const float origin[3], float* pos_xyz, uint32_t capacity,
                          uint32_t* count);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * seeding as fs_scene_seed, allocating nothing (no hipMalloc, no stream).
+ * The lattice is staged in a fixed device array of FS_SEED_SCRATCH_BYTES
+ * (one per GPU, held until the process ends) and copied to pos_xyz on the
+ * null stream; the call returns when the copy is done.  Lattices of more
+ * than FS_SEED_SCRATCH_PARTICLES particles fail with FS_ERR_INVALID_ARGUMENT.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (from several threads) race.  Same results as fs_scene_seed. */
+#define FS_SEED_SCRATCH_BYTES (128u * 1024u * 1024u)
+#define FS_SEED_SCRATCH_PARTICLES (FS_SEED_SCRATCH_BYTES / 12u)
+FS_API int fs_scene_seed_noalloc(const fs_scene* scene, float h, const float origin[3], float* pos_xyz,
+                                 uint32_t capacity, uint32_t* count);
 /* GpuFluid::new (src/gpu/fluid.rs:247-318) from a registry and a scene:
  * h = 2/64, the tank centred on the origin, the scene's particles seeded on
  * the GPU at rest, and Params from the registry with the scene's effective
