@@ -358,7 +358,13 @@ int substep_reference(fs_tank* t, bool sort_now) {
     ParticleSoA live = t->side[t->cur];
     FS_HIP(launch_classify(P, t->occ, t->ct, t->counters, s));
     FS_MARK(t, kSecClassify);
-    FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, 0u, s));
+    // With the sort enabled the reference scatters through the workgroup-local
+    // table on every substep (src/gpu/fluid.rs:519-530, 1572-1584); without
+    // it, per-tap global atomics (sh/scatter.wgsl).
+    if (t->prm.particle_sort && p2g_local_supported(P) && !env_is("FLUIDSTEP_REF_P2G", "atomic"))
+        FS_HIP(launch_p2g_local(P, live, t->bf16, t->num, t->den, s));
+    else
+        FS_HIP(launch_p2g_atomic(P, live, t->bf16, t->num, t->den, nullptr, 0u, s));
     FS_MARK(t, kSecScatter);
     for (int a = 0; a < 3; ++a) {
         FS_HIP(launch_normalize(P, a, t->num[a], t->den[a], t->vel[a], s));

@@ -70,6 +70,13 @@ hipError_t launch_clear_faces_guarded(const StepParams& P, int32_t* const num[3]
 hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf16,
                              int32_t* const num[3], int32_t* const den[3], const uint32_t* guard,
                              uint32_t above, hipStream_t s);
+// sh/scatter_local.wgsl: workgroup-local hash pre-accumulation, one global
+// atomic per touched face slot (same num/den bits as launch_p2g_atomic).
+// Supported when every face index fits the key below 2^29 (else use the
+// per-tap form).
+bool p2g_local_supported(const StepParams& P);
+hipError_t launch_p2g_local(const StepParams& P, const ParticleSoA& src, int bf16, int32_t* const num[3],
+                            int32_t* const den[3], hipStream_t s);
 // Reference-structured per-axis passes (one kernel each, sh/*.wgsl).
 hipError_t launch_normalize(const StepParams& P, int a, const int32_t* num, const int32_t* den,
                             float* vel, hipStream_t s);

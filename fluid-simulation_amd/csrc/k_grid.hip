@@ -74,6 +74,119 @@ hipError_t launch_p2g_atomic(const StepParams& P, const ParticleSoA& src, int bf
     return hipGetLastError();
 }
 
+// sh/scatter_local.wgsl:149-198: the reference's P2G on every substep when
+// the particle sort is enabled (src/gpu/fluid.rs:519-530, 1572-1584).  A
+// 64-lane workgroup accumulates its particles' taps in a workgroup-local
+// open-addressed table (kLocalCap slots, linear probing, key = buffer <<
+// shift | face index, stored +1 so that 0 marks an empty slot), then adds each
+// occupied slot to global memory with one atomic; a tap that finds the table
+// full goes to global memory directly.  All sums are wrapping i32 adds, so
+// num/den equal the per-tap atomic form's bit for bit.
+constexpr uint32_t kLocalCap = 1024;  // CAP of sh/scatter_local.wgsl:49
+
+__device__ __forceinline__ void add_global6(uint32_t buf, uint32_t idx, int32_t v, int32_t* un, int32_t* ud,
+                                            int32_t* vn, int32_t* vd, int32_t* wn, int32_t* wd) {
+    switch (buf) {
+        case 0: atomicAdd(&un[idx], v); break;
+        case 1: atomicAdd(&ud[idx], v); break;
+        case 2: atomicAdd(&vn[idx], v); break;
+        case 3: atomicAdd(&vd[idx], v); break;
+        case 4: atomicAdd(&wn[idx], v); break;
+        default: atomicAdd(&wd[idx], v); break;
+    }
+}
+
+template <int BF16>
+__global__ __launch_bounds__(64) void k_p2g_local(StepParams P, ParticleSoA sp, int32_t* __restrict__ un,
+                                                  int32_t* __restrict__ ud, int32_t* __restrict__ vn,
+                                                  int32_t* __restrict__ vd, int32_t* __restrict__ wn,
+                                                  int32_t* __restrict__ wd, uint32_t shift) {
+    __shared__ uint32_t s_key[kLocalCap];
+    __shared__ int32_t s_val[kLocalCap];
+    for (uint32_t i = threadIdx.x; i < kLocalCap; i += 64) {
+        s_key[i] = 0u;
+        s_val[i] = 0;
+    }
+    __syncthreads();
+    auto local_add = [&](uint32_t buf, uint32_t idx, int32_t v) {
+        const uint32_t key = (buf << shift) | idx;
+        const uint32_t k1 = key + 1u;
+        uint32_t h = key * 2654435761u;  // Knuth's multiplicative hash, then xor-shift
+        h = (h ^ (h >> 15)) & (kLocalCap - 1u);
+        for (uint32_t probe = 0; probe < kLocalCap; ++probe) {
+            const uint32_t cur = __hip_atomic_load(&s_key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cur == k1) {
+                atomicAdd(&s_val[h], v);
+                return;
+            }
+            if (cur == 0u) {
+                const uint32_t old = atomicCAS(&s_key[h], 0u, k1);
+                if (old == 0u || old == k1) {
+                    atomicAdd(&s_val[h], v);
+                    return;
+                }
+            }
+            h = (h + 1u) & (kLocalCap - 1u);
+        }
+        add_global6(buf, idx, v, un, ud, vn, vd, wn, wd);
+    };
+    const uint32_t p = blockIdx.x * 64u + threadIdx.x;
+    if (p < P.N) {
+        const float pos[3] = {(sp.pos[0][p] - P.origin[0]) * P.inv_h, (sp.pos[1][p] - P.origin[1]) * P.inv_h,
+                              (sp.pos[2][p] - P.origin[2]) * P.inv_h};
+        const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
+                             load_vel<BF16>(sp.vel[2], p)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
+            p2g_taps(g, dim, pv[a], P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
+                const uint32_t idx = (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk);
+                local_add(2u * a, idx, tn);
+                local_add(2u * a + 1u, idx, td);
+            });
+        }
+    }
+    __syncthreads();
+    const uint32_t mask = (1u << shift) - 1u;
+    for (uint32_t i = threadIdx.x; i < kLocalCap; i += 64) {
+        const uint32_t k1 = s_key[i];
+        if (k1 == 0u) continue;
+        const int32_t v = s_val[i];
+        if (v == 0) continue;
+        const uint32_t key = k1 - 1u;
+        add_global6(key >> shift, key & mask, v, un, ud, vn, vd, wn, wd);
+    }
+}
+
+bool p2g_local_supported(const StepParams& P) {
+    uint32_t fmax = P.F[0] > P.F[1] ? P.F[0] : P.F[1];
+    fmax = fmax > P.F[2] ? fmax : P.F[2];
+    uint32_t shift = 0;
+    while ((1ull << shift) < fmax) ++shift;
+    // six buffers << shift, plus one, must stay below 2^32
+    return shift <= 29u;
+}
+
+hipError_t launch_p2g_local(const StepParams& P, const ParticleSoA& src, int bf16, int32_t* const num[3],
+                            int32_t* const den[3], hipStream_t s) {
+    if (P.N == 0) return hipSuccess;
+    uint32_t fmax = P.F[0] > P.F[1] ? P.F[0] : P.F[1];
+    fmax = fmax > P.F[2] ? fmax : P.F[2];
+    uint32_t shift = 0;
+    while ((1ull << shift) < fmax) ++shift;
+    if (shift > 29u) return hipErrorInvalidValue;
+    const unsigned g = grid_for(P.N, 64);
+    if (bf16)
+        FS_LAUNCH(k_p2g_local<1>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2], den[2],
+                  shift);
+    else
+        FS_LAUNCH(k_p2g_local<0>, dim3(g), dim3(64), 0, s, P, src, num[0], den[0], num[1], den[1], num[2], den[2],
+                  shift);
+    return hipGetLastError();
+}
+
 // Clears num and den of the three axes when *guard > above (the fused
 // engine's fallback to the atomic P2G): one launch for the six arrays.
 __global__ void k_clear_faces_guarded(StepParams P, int32_t* __restrict__ n0, int32_t* __restrict__ n1,

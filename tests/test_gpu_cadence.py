@@ -6,7 +6,8 @@ engines, against the CPU restatement run with the same sort_period and tick.
 On a substep without a sort the reference keeps the particle order
 (sh/scatter.wgsl / sh/g2p.wgsl run over the live buffer as it lies); the
 fused engine then gathers P2G through a cell order of the live side and the
-reference-structured engine scatters with atomics.  Both must give the
+reference-structured engine scatters through the workgroup-local table of
+sh/scatter_local.wgsl (per-tap atomics with the sort off).  Both must give the
 restatement's integer sums bit for bit (`saved`), its particle order, and
 faces, pressure and particles within 1e-5 relative L2 per substep.
 
@@ -211,3 +212,37 @@ def test_unsorted_substeps_keep_the_last_sorts_bin_starts():
     assert np.array_equal(t.cell_offset(), first)
     assert not np.array_equal(t.occupancy(), occ0)
     t.close()
+
+
+def test_reference_engine_local_scatter_with_a_full_table(monkeypatch):
+    """The reference-structured engine scatters through the workgroup-local
+    table (sh/scatter_local.wgsl) whenever the sort is enabled.  Particles in
+    random order on a substep without a sort make a 64-particle workgroup's
+    taps overflow the 1024-slot table, so the rest go to global memory
+    directly; the integer sums still equal the restatement's, and the per-tap
+    atomic form's (FLUIDSTEP_REF_P2G=atomic), bit for bit."""
+    sc = scenes.small_tank(n=24, seed=5, fill=0.5)
+    rng = np.random.default_rng(17)
+    order = rng.permutation(len(sc["pos"]))
+    pos, vel = sc["pos"][order].copy(), sc["vel"][order].copy()
+    pr = np.zeros(24 ** 3, np.float32)
+    P = oracle_params(sc, 0)
+    prm = gpu_params(sc, fsm.ENGINE_REFERENCE, particle_sort=1, particle_sort_period=4)
+    outs = []
+    for form in ("local", "atomic"):
+        if form == "atomic":
+            monkeypatch.setenv("FLUIDSTEP_REF_P2G", "atomic")
+        t = fsm.FluidStep(prm, pos, vel)
+        t.step(1)  # tick 0 sorts; tick 1 does not
+        t.load_state(pos, vel, pr)
+        t.step(1)
+        say("full local table", form, "N", len(pos))
+        assert t.stats()["sorted"] == 0
+        outs.append((t.faces(fsm.FACE_SAVED), t.pressure(), *t.particles()))
+        t.close()
+    ref = oracle.run(P, pos, vel, pressure=pr, steps=1, sort_period=4, tick0=1)
+    for saved, p, gp, gv in outs:
+        for a in range(3):
+            assert bits_equal(saved[a], ref.saved[a]), a
+        assert bits_equal(p, ref.pressure)
+        assert bits_equal(gp, ref.pos) and bits_equal(gv, ref.vel_particles)
