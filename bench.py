@@ -49,6 +49,25 @@ def load_traffic():
     return None
 
 
+def copy_calibration(mib):
+    """A plain elementwise copy of known size (torch.neg of `mib` MiB of f32
+    into another buffer: that many bytes read and written; the source, larger
+    than the L2 and Infinity Cache together, is not resident), run before the
+    workload so that a counter run of this command reads the same counters
+    for a kernel of known traffic and can scale the others by it."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"kernel": "torch.neg (elementwise)", "dispatches": 3,
+                                           "bytes_read": 4 * n, "bytes_written": 4 * n}}), file=sys.stderr, flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def run_gpu(fsm, sc, params, steps, warmup, timing_steps=3):
     pos, vel = sc["pos"], sc["vel"]
     t = fsm.FluidStep(params, pos, vel)
@@ -85,16 +104,27 @@ def fine_sections(d):
     return out
 
 
-def cpu_baseline(sc, steps):
+def cpu_baseline(sc, steps, threads=None):
+    """The plain-FP32 CPU restatement over `steps` substeps from the
+    workload's initial state: (seconds, cores used, ms per phase per substep).
+    threads=None uses every OpenMP thread of the host."""
     import oracle
 
     import helpers  # tests/helpers.py
 
     P = helpers.oracle_params(sc, 0)
+    full = oracle.num_threads()
+    if threads:
+        oracle.set_num_threads(threads)
+    cores = oracle.num_threads()
+    oracle.phase_reset()
     t0 = time.perf_counter()
-    res = oracle.run(P, sc["pos"], sc["vel"], steps=steps, sort_period=1, want=False)
+    oracle.run(P, sc["pos"], sc["vel"], steps=steps, sort_period=1, want=False)
     dt = time.perf_counter() - t0
-    return dt, res, oracle.num_threads()
+    phases = {k: round(v / steps, 3) for k, v in oracle.phase_ms().items()}
+    if threads:
+        oracle.set_num_threads(full)
+    return dt, cores, phases
 
 
 def _pct(xs, q):
@@ -539,6 +569,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps-1t", type=int, default=3, help="substeps of the one-thread CPU row (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--bf16", action="store_true", help="bf16 particle-velocity storage")
@@ -563,6 +594,8 @@ def main():
                     choices=["high_density_128", "large_splash_256", "dam_break_32", "control_64", "batched_tanks"],
                     help="the metric is quoted on high_density_128; the others are for per-kernel profiles "
                          "and the batched-tank figures")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run an elementwise copy of MIB MiB before the workload (counter-run calibration)")
     ap.add_argument("--no-tune", action="store_true", help="batched_tanks: skip the autotuner")
     ap.add_argument("--latency-steps", type=int, default=20, help="batched_tanks: steps timed one by one")
     ap.add_argument("--seq-steps", type=int, default=3, help="batched_tanks: steps of the sequential baseline")
@@ -607,6 +640,9 @@ def main():
     if args.workload == "batched_tanks":
         print(json.dumps(batch_bench(fsm, scenes, gpu_params, args)), flush=True)
         return
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     sc = getattr(scenes, args.workload)()
     N = len(sc["pos"])
@@ -655,7 +691,14 @@ def main():
             "flip": sc["physics"]["flip_blend"], "cg_tolerance": 1e-5, "cg_cap": 200, "sort_period": 1, "engine": "fused",
         },
         "cg": {"iterations_per_solve": iters, "active_cells": act, "ms_per_solve": cg_ms,
-               "ms_per_iteration": [m / max(k, 1) for m, k in zip(cg_ms, iters)]},
+               "ms_per_iteration": [m / max(k, 1) for m, k in zip(cg_ms, iters)],
+               "relative_residual_target": 1e-5,
+               # |r_k| / |r_0| of the recurrence (the stop test) and |b - A p| / |r_0|
+               # recomputed in f64 after the solve, per timed solve
+               "relative_residual": [float(np.sqrt(s["rs_final"] / s["rs_initial"])) if s["rs_initial"] > 0
+                                     else 0.0 for s in per],
+               "relative_true_residual": [float(np.sqrt(s["true_residual_sq"] / s["rs_initial"]))
+                                          if s["rs_initial"] > 0 else 0.0 for s in per]},
         "sections_ms": {k: per[-1][k] for k in ("ms_prep", "ms_pressure", "ms_finish", "ms_cg")},
         "fine_sections_ms": fine_sections(per[-1]["detailed"]),
         "dispatches_per_substep": per[-1]["dispatches_per_substep"],
@@ -669,6 +712,9 @@ def main():
             "basis": "45 B/cell/iteration + 30 B/cell/solve over the cells of liquid tiles",
             "achieved_full_grid_basis": float(np.mean(ach_full)) if ach_full else None,
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "traffic_source": ("stored figure, not measured in this run: " + traffic.get("source", "")
+                               + " (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, each scaled by the same "
+                               "counter's reading for a copy of known size)") if traffic else None,
         },
     }
 
@@ -681,12 +727,21 @@ def main():
         line["speedup_vs_plain_fp32"] = value / pv
 
     if not args.no_cpu:
-        cdt, res, cores = cpu_baseline(sc, args.cpu_steps)
+        cdt, cores, phases = cpu_baseline(sc, args.cpu_steps)
         line["cpu_baseline"] = {
             "value": N * args.cpu_steps / cdt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"{args.cpu_steps} substeps of the {sc['name']} workload from its initial state, plain FP32 "
                       f"restatement (oracle/), OpenMP {cores} threads, {cdt:.1f} s",
+            "ms_per_substep_by_phase": phases,
         }
+        if args.cpu_steps_1t > 0:
+            c1, _, ph1 = cpu_baseline(sc, args.cpu_steps_1t, threads=1)
+            line["cpu_baseline_1_thread"] = {
+                "value": N * args.cpu_steps_1t / c1, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                "sample": f"{args.cpu_steps_1t} substeps of the {sc['name']} workload from its initial state, "
+                          f"plain FP32 restatement, one thread, {c1:.1f} s",
+                "ms_per_substep_by_phase": ph1,
+            }
     print(json.dumps(line), flush=True)
 
 

@@ -11,9 +11,12 @@ creating a tank fails with FluidStepError when no HIP device is present.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import math
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -365,6 +368,35 @@ def make_params(nx, ny, nz, h, origin=(0.0, 0.0, 0.0), **kw) -> Params:
     return p
 
 
+# Handles that own GPU resources (frame drivers, tanks, batches), closed
+# explicitly by an exit hook while the HIP runtime is still up: Python's exit
+# hooks run before the process's own teardown, whereas a finalizer left to the
+# interpreter's shutdown could reach the library after the runtime is gone.
+_live_frames = weakref.WeakSet()
+_live_gpu = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for group in (_live_frames, _live_gpu):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+def _finalize(obj):
+    """__del__ of a GPU handle: close it, except during interpreter shutdown
+    (the exit hook has closed every live handle by then)."""
+    if sys is None or sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
 class Registry:
     """The settings registry (host only)."""
 
@@ -471,13 +503,15 @@ class Frame:
         h = C.c_void_p()
         _check(lib().fs_frame_create(tank._h, settings._h, C.byref(h)))
         self._h, self.tank, self.settings = h, tank, settings
+        _live_frames.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
             lib().fs_frame_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        _finalize(self)
 
     def run(self, render_dt_ms: float) -> dict:
         st = TimestepStats()
@@ -577,6 +611,7 @@ class FluidStep:
         _check(lib().fs_get_params(h, C.byref(self.params)))
         self.n = int(lib().fs_particle_count(h))
         self.dims = (self.params.nx, self.params.ny, self.params.nz)
+        _live_gpu.add(self)
         return self
 
     def __init__(self, params: Params, pos, vel=None):
@@ -588,13 +623,15 @@ class FluidStep:
         self.params = params
         self.n = len(pos)
         self.dims = (params.nx, params.ny, params.nz)
+        _live_gpu.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
             lib().fs_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        _finalize(self)
 
     def __enter__(self):
         return self
@@ -774,13 +811,15 @@ class FluidBatch:
         self.params = list(params_list)
         self.counts = [len(p) for p in self._pos]
         self._pos = self._vel = None
+        _live_gpu.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
             lib().fs_batch_destroy(self._h)
             self._h = None
 
-    __del__ = close
+    def __del__(self):
+        _finalize(self)
 
     def __enter__(self):
         return self

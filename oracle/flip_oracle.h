@@ -161,6 +161,13 @@ void orc_run_frozen_f64(const orc_params *P, uint32_t n, double *const pos[3], d
                         double *pressure, uint32_t steps, const uint32_t *ct_steps, const uint32_t *occ_steps);
 void *orc_adj_tape_from_states_f64(const orc_params *P, uint32_t n, uint32_t steps, const double *states);
 
+/* Per-phase wall time (ms) of orc_run_*, summed since the last reset: mark,
+ * sort, classify, scatter, face epilogue, divergence, CG, gradient, G2P. */
+void orc_phase_ms_f32(double *out9);
+void orc_phase_reset_f32(void);
+void orc_phase_ms_f64(double *out9);
+void orc_phase_reset_f64(void);
+
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 

@@ -632,6 +632,22 @@ static void permute(uint32_t n, const uint32_t *perm, real *a, real *tmp) {
     memcpy(a, tmp, sizeof(real) * n);
 }
 
+/* Wall time of each phase of FN(run), summed over its substeps since the
+ * last reset (the CPU baseline's per-phase figures): mark, sort, classify,
+ * scatter (P2G), face epilogue (normalize .. enforce), divergence, CG,
+ * gradient + enforce, G2P. */
+#define ORC_PHASES 9
+static double CAT(g_phase_ms_, SFX)[ORC_PHASES];
+static inline double orc_now_ms(void) {
+#ifdef _OPENMP
+    return omp_get_wtime() * 1e3;
+#else
+    return 0.0;
+#endif
+}
+void FN(phase_ms)(double *out) { memcpy(out, CAT(g_phase_ms_, SFX), sizeof(double) * ORC_PHASES); }
+void FN(phase_reset)(void) { memset(CAT(g_phase_ms_, SFX), 0, sizeof(double) * ORC_PHASES); }
+
 /* One or more substeps in the order of src/gpu/mod.rs:993-1033. */
 int FN(run)(const orc_params *P, uint32_t n, real *px, real *py, real *pz, real *vx, real *vy,
             real *vz, real *pressure, uint32_t steps, uint32_t sort_period, uint32_t tick0,
@@ -660,10 +676,13 @@ int FN(run)(const orc_params *P, uint32_t n, real *px, real *py, real *pz, real 
     orc_stats st;
     memset(&st, 0, sizeof st);
     int sorted_last = 0;
+    double *ph = CAT(g_phase_ms_, SFX);
     for (uint32_t s = 0; s < steps; ++s) {
+        double t0 = orc_now_ms(), t1;
         /* clear (pressure_a only when warm start is off) + mark */
         if (!P->warm_start) memset(pressure, 0, sizeof(real) * C);
         FN(histogram)(P, n, px, py, pz, occ);
+        t1 = orc_now_ms(); ph[0] += t1 - t0; t0 = t1;
         const uint32_t tick = tick0 + s;
         sorted_last = sort_period > 0 && (tick % sort_period) == 0u;
         if (sorted_last) {
@@ -677,9 +696,13 @@ int FN(run)(const orc_params *P, uint32_t n, real *px, real *py, real *pz, real 
             permute(n, perm, vz, tmp);
             if (out && out->cell_offset) orc_exclusive_scan_u32(C, occ, out->cell_offset);
         }
+        t1 = orc_now_ms(); ph[1] += t1 - t0; t0 = t1;
         st.liquid_cells = classify(P, occ, ct);
+        t1 = orc_now_ms(); ph[2] += t1 - t0; t0 = t1;
         p2g(P, n, px, py, pz, vx, vy, vz, num, den);
+        t1 = orc_now_ms(); ph[3] += t1 - t0; t0 = t1;
         p2g_epilogue(P, ct, num, den, vel, saved);
+        t1 = orc_now_ms(); ph[4] += t1 - t0; t0 = t1;
         if (out) {
             for (int a = 0; a < 3; ++a) {
                 if (out->num[a]) memcpy(out->num[a], num[a], sizeof(ACC) * fc[a]);
@@ -688,10 +711,15 @@ int FN(run)(const orc_params *P, uint32_t n, real *px, real *py, real *pz, real 
                 if (out->saved[a]) memcpy(out->saved[a], saved[a], sizeof(real) * fc[a]);
             }
         }
+        t0 = orc_now_ms();
         divergence(P, vel, ct, occ, div);
+        t1 = orc_now_ms(); ph[5] += t1 - t0; t0 = t1;
         FN(pressure_solve)(P, ct, div, pressure, &st);
+        t1 = orc_now_ms(); ph[6] += t1 - t0; t0 = t1;
         gradient_enforce(P, pressure, ct, vel);
+        t1 = orc_now_ms(); ph[7] += t1 - t0; t0 = t1;
         g2p(P, n, px, py, pz, vx, vy, vz, vel, saved);
+        t1 = orc_now_ms(); ph[8] += t1 - t0;
     }
     if (out) {
         if (out->occupancy) memcpy(out->occupancy, occ, sizeof(uint32_t) * C);

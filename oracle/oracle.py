@@ -328,6 +328,20 @@ def run(P, pos, vel, pressure=None, steps=1, sort_period=1, tick0=0, f64=False, 
     return res
 
 
+PHASES = ("mark", "sort", "classify", "scatter", "face_epilogue", "divergence", "cg", "gradient", "g2p")
+
+
+def phase_ms(f64=False) -> dict:
+    """Wall time per phase of run() since the last phase_reset (ms)."""
+    out = np.zeros(len(PHASES), np.float64)
+    getattr(lib(), "orc_phase_ms_f64" if f64 else "orc_phase_ms_f32")(ptr(out))
+    return dict(zip(PHASES, out.tolist()))
+
+
+def phase_reset(f64=False) -> None:
+    getattr(lib(), "orc_phase_reset_f64" if f64 else "orc_phase_reset_f32")()
+
+
 def num_threads() -> int:
     return int(lib().orc_num_threads())
 
