@@ -302,6 +302,52 @@ def test_full_size_high_density_one_step():
     t.close()
 
 
+@pytest.mark.timeout(900)
+def test_full_size_large_splash_256():
+    """256^3, pool plus falling sphere (35,450,304 particles): one
+    teacher-forced step of the fused engine against the restatement, then
+    two more free-running substeps under size-independent invariants
+    (occupancy sums to N, bin starts are the scan of the occupancy, the
+    permutation is a bijection that orders the particles by cell, the state
+    stays finite and inside the tank)."""
+    sc = scenes.large_splash_256()
+    N = len(sc["pos"])
+    n = sc["n"]
+    say("full size", sc["name"], "N", N)
+    ref, t = run_pair(sc, fsm.ENGINE_FUSED)
+    say("large splash: restatement step done, cg", ref.stats.cg_iterations)
+    assert np.array_equal(t.occupancy(), ref.occupancy)
+    assert np.array_equal(t.permutation(), ref.perm)
+    assert np.array_equal(t.cell_type(), ref.cell_type)
+    saved = t.faces(fsm.FACE_SAVED)
+    for a in range(3):
+        assert bits_equal(saved[a], ref.saved[a]), a
+    st = t.stats()
+    assert abs(st["cg_iterations"] - ref.stats.cg_iterations) <= 1
+    assert rel_l2(t.pressure(), ref.pressure) <= TOL
+    gp, gv = t.particles()
+    assert rel_l2(gp, ref.pos) <= TOL and rel_l2(gv, ref.vel_particles) <= TOL
+    del ref, saved
+    h = np.float32(1.0) / np.float32(n)
+    for s in range(2):
+        t.step(1)
+        say("large splash: free substep", s)
+        occ = t.occupancy()
+        assert int(occ.sum(dtype=np.uint64)) == N
+        off = t.cell_offset()
+        assert np.array_equal(off, np.concatenate([[0], np.cumsum(occ, dtype=np.uint64)[:-1]]).astype(np.uint32))
+        perm = t.permutation()
+        seen = np.zeros(N, np.bool_)
+        seen[perm] = True
+        assert seen.all()
+        gp, gv = t.particles()
+        assert np.all(np.isfinite(gp)) and np.all(np.isfinite(gv))
+        lo, hi = np.float32(1.05) * h, np.float32(1.0) - np.float32(1.05) * h
+        assert gp.min() >= lo - 1e-6 and gp.max() <= hi + 1e-6
+        assert np.all(np.isfinite(t.pressure()))
+    t.close()
+
+
 @pytest.mark.parametrize("cap", [30, 31])
 def test_streaming_solve_parity(cap, monkeypatch):
     """The persistent solve's streaming form (a workgroup walks its tiles in
