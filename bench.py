@@ -207,126 +207,128 @@ def batch_bench(fsm, scenes, gpu_params, args):
     return line
 
 
-def adjoint_report(fsm, scenes, gpu_params, steps=60, every=10, h=1e-3, fp64=True):
+def adjoint_report(fsm, scenes, gpu_params, horizons=(10, 20, 60), every=10, hs=(1e-3, 1e-5), fp64=True,
+                   tolerance=None):
     """Differentiable control at 64^3: gradient of |centroid - (0.7,0.3,0.5)|^2
-    after `steps` substeps w.r.t. the initial particle velocities and gravity_y,
-    from the checkpointed forward + reverse pass on the GPU; checked against
-    central differences of the GPU forward (h) along uniform velocity shifts
-    and gravity, and against the FP64 reverse pass of the restatement."""
+    after H substeps w.r.t. a uniform shift of the initial particle velocity
+    (x, y, z) and gravity_y, for each horizon H.
+
+    gpu:        the checkpointed forward + reverse pass on the GPU.
+    fp64:       the FP64 reverse pass of the restatement along the GPU's own
+                trajectory (its input state of every substep).
+    frozen_fd:  central differences (each h) of the smooth forward with every
+                decision of the GPU trajectory replayed (sort order, cell types,
+                occupancies, face supports, CFL and wall-clamp branches; stencils
+                from the current positions), real P2G sums and an exact pressure
+                solve: the function the adjoint's conventions define.
+    plain_fd:   central differences (h = 1e-3) of the GPU forward itself, whose
+                discrete decisions move with the perturbation."""
+    import oracle
+
+    import helpers
+
     sc = scenes.control_64()
+    if tolerance is not None:  # a tighter pressure solve for the forward (cap raised to match)
+        sc["physics"]["pressure_residual_tolerance"] = tolerance
+        sc["physics"]["pressure_iterations"] = 2000
     target = np.array([0.7, 0.3, 0.5])
     params = gpu_params(sc, fsm.ENGINE_FUSED)
-
-    def upstream(pos):
-        c = pos.astype(np.float64).mean(0)
-        return np.tile(2.0 * (c - target) / len(pos), (len(pos), 1))
+    P64 = helpers.oracle_params(sc, 1)
+    v0 = sc["vel"].astype(np.float64)
+    g0 = float(params.gravity[1])
 
     def loss(pos):
         c = pos.astype(np.float64).mean(0)
         return float(((c - target) ** 2).sum())
 
+    def upstream(pos):
+        c = pos.astype(np.float64).mean(0)
+        return np.tile(2.0 * (c - target) / len(pos), (len(pos), 1))
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    def grad4(gv, gg):
+        return [float(gv[:, 0].sum()), float(gv[:, 1].sum()), float(gv[:, 2].sum()), float(gg[1])]
+
     t = fsm.FluidStep(params, sc["pos"], sc["vel"])
     t.forward_checkpointed(1, 1)  # warm-up of every kernel
     t.close()
-    t = fsm.FluidStep(params, sc["pos"], sc["vel"])
-    t0 = time.perf_counter()
-    t.forward_checkpointed(steps, every)
-    t.synchronize()
-    fwd_s = time.perf_counter() - t0
-    pos, _ = t.particles()
-    L0 = loss(pos)
-    t0 = time.perf_counter()
-    gx, gv, gg = t.adjoint(upstream(pos), None)
-    bwd_s = time.perf_counter() - t0
-    t.close()
-
-    def L_at(vel, gy):
-        p = fsm.Params()
-        import ctypes
-        ctypes.pointer(p)[0] = params
-        p.gravity[1] = gy
-        tt = fsm.FluidStep(p, sc["pos"], vel.astype(np.float32))
-        tt.step(steps)
-        pp, _ = tt.particles()
-        tt.close()
-        return loss(pp)
-
-    checks = {}
-    v0 = sc["vel"].astype(np.float64)
-    for axis, name in enumerate("xyz"):
-        d = np.zeros_like(v0)
-        d[:, axis] = 1.0
-        fd = (L_at(v0 + h * d, -9.81) - L_at(v0 - h * d, -9.81)) / (2 * h)
-        ad = float((gv.astype(np.float64) * d).sum())
-        checks[f"vel_shift_{name}"] = {"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / max(abs(fd), 1e-30)}
-    fd = (L_at(v0, -9.81 + h) - L_at(v0, -9.81 - h)) / (2 * h)
-    checks["gravity_y"] = {"fd": fd, "adjoint": float(gg[1]), "rel_err": abs(fd - gg[1]) / max(abs(fd), 1e-30)}
-    report = {
-        "workload": "control_64", "particles": len(sc["pos"]), "steps": steps, "checkpoint_every": every,
-        "loss": L0, "gpu_forward_checkpointed_s": fwd_s, "gpu_adjoint_s": bwd_s,
-        "fd_h": h, "fd_checks_gpu_forward": checks,
-        "grad_norms": {"vel": float(np.linalg.norm(gv)), "pos": float(np.linalg.norm(gx)), "gravity": gg.tolist()},
-    }
-    if fp64:
-        import oracle
-
-        import helpers
-
-        rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-        P = helpers.oracle_params(sc, 1)
-        # (1) The FP64 reverse pass along the GPU's own trajectory: the GPU
-        # forward's input state of every substep (read-out order, warm-start
-        # pressure), then the FP64 adjoint of each substep from that state.
+    report = {"workload": "control_64", "particles": len(sc["pos"]), "checkpoint_every": every,
+              "cg_tolerance": float(sc["physics"]["pressure_residual_tolerance"]),
+              "cg_cap": int(sc["physics"]["pressure_iterations"]),
+              "gradient": "d loss / d (uniform initial velocity shift x, y, z; gravity_y)", "horizons": []}
+    for H in horizons:
+        ent = {"steps": H}
+        t = fsm.FluidStep(params, sc["pos"], sc["vel"])
         t0 = time.perf_counter()
-        tt = fsm.FluidStep(params, sc["pos"], sc["vel"])
-        states = []
-        for _ in range(steps):
-            p_, v_ = tt.particles()
-            states.append((p_, v_, tt.pressure()))
-            tt.step(1)
-        pf, _ = tt.particles()
-        tt.close()
-        T = oracle.AdjointTape.from_states(P, states)
-        del states
-        tgx, tgv, tgg = T.backward(upstream(pf), None)
-        T.close()
-        report["fp64_on_gpu_trajectory"] = {
-            "seconds": time.perf_counter() - t0,
-            "rel_l2_vel_grad": rel(gv.astype(np.float64), tgv), "rel_l2_pos_grad": rel(gx.astype(np.float64), tgx),
-            "gravity": tgg.tolist(), "rel_err_gravity": float(np.linalg.norm(gg - tgg) / max(np.linalg.norm(tgg), 1e-300)),
-        }
-        # (2) The FP64 forward and reverse pass on their own trajectory, and
-        # central differences (h) of the FP64 forward with the base run's
-        # cell types and occupancies replayed (the function the adjoint
-        # differentiates: the plain forward's difference also carries the
-        # jumps of surface cells that flip between the +h and -h runs).
+        t.forward_checkpointed(H, every)
+        t.synchronize()
+        fwd_s = time.perf_counter() - t0
+        pos, _ = t.particles()
         t0 = time.perf_counter()
-        T = oracle.AdjointTape(P, sc["pos"].astype(np.float64), v0, steps=steps, every=every, f64=True, record=True)
-        ogx, ogv, ogg = T.backward(upstream(T.pos), None)
-        frozen = {}
+        gx, gv, gg = t.adjoint(upstream(pos), None)
+        bwd_s = time.perf_counter() - t0
+        t.close()
+        g_gpu = grad4(gv, gg)
+        ent["gpu"] = {"loss": loss(pos), "grad": g_gpu, "forward_checkpointed_s": fwd_s, "adjoint_s": bwd_s}
+        print(f"adjoint report: H={H} gpu grad {g_gpu}", file=sys.stderr, flush=True)
 
-        def Lf(vel, gy):
-            Pg = helpers.oracle_params(sc, 1)
-            Pg.grav[1] = gy
-            pos_, _, _ = oracle.run_frozen(Pg, sc["pos"].astype(np.float64), vel, T.ct_steps, T.occ_steps)
-            return loss(pos_)
+        def L_gpu(vel, gy):
+            p = fsm.Params.from_buffer_copy(params)
+            p.gravity[1] = gy
+            tt = fsm.FluidStep(p, sc["pos"], vel.astype(np.float32))
+            tt.step(H)
+            pp, _ = tt.particles()
+            tt.close()
+            return loss(pp)
 
-        for axis, name in enumerate("xyz"):
-            d = np.zeros_like(v0)
-            d[:, axis] = 1.0
-            fd = (Lf(v0 + h * d, -9.81) - Lf(v0 - h * d, -9.81)) / (2 * h)
-            ad = float((ogv * d).sum())
-            frozen[f"vel_shift_{name}"] = {"fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / max(abs(fd), 1e-30)}
-        fd = (Lf(v0, -9.81 + h) - Lf(v0, -9.81 - h)) / (2 * h)
-        frozen["gravity_y"] = {"fd": fd, "adjoint": float(ogg[1]), "rel_err": abs(fd - ogg[1]) / max(abs(fd), 1e-30)}
-        report["fp64_reference"] = {
-            "seconds": time.perf_counter() - t0, "threads": oracle.num_threads(), "loss": loss(T.pos),
-            "frozen_decision_fd": frozen,
-            "gpu_vs_fp64_own_trajectories": {
-                "rel_l2_vel_grad": rel(gv.astype(np.float64), ogv), "rel_l2_pos_grad": rel(gx.astype(np.float64), ogx),
-                "gravity_y": float(ogg[1]), "rel_err_gravity_y": abs(gg[1] - ogg[1]) / max(abs(ogg[1]), 1e-30)},
-        }
-        T.close()
+        hp = 1e-3
+        fd = [(L_gpu(v0 + hp * np.eye(3)[a], g0) - L_gpu(v0 - hp * np.eye(3)[a], g0)) / (2 * hp) for a in range(3)]
+        fd.append((L_gpu(v0, g0 + hp) - L_gpu(v0, g0 - hp)) / (2 * hp))
+        ent["plain_fd"] = {"h": hp, "grad": fd, "rel_err_vs_gpu": rel(g_gpu, fd)}
+        if fp64:
+            # The GPU trajectory: every substep's input state (read-out order).
+            tt = fsm.FluidStep(params, sc["pos"], sc["vel"])
+            states = []
+            for _ in range(H):
+                p_, v_ = tt.particles()
+                states.append((p_, v_, tt.pressure()))
+                tt.step(1)
+            pf, _ = tt.particles()
+            tt.close()
+            t0 = time.perf_counter()
+            T = oracle.AdjointTape.from_states(P64, states)
+            _, ogv, ogg = T.backward(upstream(pf), None)
+            T.close()
+            g64 = grad4(ogv, ogg)
+            ent["fp64"] = {"grad": g64, "seconds": time.perf_counter() - t0, "rel_err_gpu_vs_fp64": rel(g_gpu, g64),
+                           "rel_l2_per_particle_vel_grad": rel(gv, ogv)}
+            rec = oracle.smooth_record_states(P64, states)
+            del states
+            x0 = sc["pos"].astype(np.float64)
+
+            def L_frozen(vel, gy):
+                Pg = oracle.make_params(P64.nx, P64.ny, P64.nz, P64.h, dt=P64.dt, flip_blend=P64.flip_blend,
+                                        grav=(0.0, gy, 0.0), rest=P64.rest, stiff=P64.stiff,
+                                        drift_clamp=P64.drift_clamp, cfl=P64.cfl,
+                                        surface_dilation=P64.surface_dilation, reduce_mode=1)
+                p_, _, _ = oracle.smooth_forward(Pg, x0, vel, rec, free_stencils=True)
+                print(f"adjoint report: H={H} frozen forward done", file=sys.stderr, flush=True)
+                return loss(p_) if np.all(np.isfinite(p_)) else float("nan")
+
+            ent["frozen_fd"] = {}
+            for h in hs:
+                t0 = time.perf_counter()
+                fd = [(L_frozen(v0 + h * np.eye(3)[a], g0) - L_frozen(v0 - h * np.eye(3)[a], g0)) / (2 * h)
+                      for a in range(3)]
+                fd.append((L_frozen(v0, g0 + h) - L_frozen(v0, g0 - h)) / (2 * h))
+                ent["frozen_fd"][f"{h:g}"] = {"grad": fd, "rel_err_gpu_vs_fd": rel(g_gpu, fd),
+                                              "rel_err_fp64_vs_fd": rel(g64, fd), "seconds": time.perf_counter() - t0}
+                print(f"adjoint report: H={H} frozen fd h={h:g} {fd}", file=sys.stderr, flush=True)
+            del rec
+        report["horizons"].append(ent)
     return report
 
 
@@ -580,6 +582,10 @@ def main():
     ap.add_argument("--adjoint-report", type=int, default=0, metavar="STEPS",
                     help="instead of timing, run the differentiable-control check at 64^3 over STEPS substeps")
     ap.add_argument("--no-fp64", action="store_true", help="adjoint report: skip the FP64 reference gradient")
+    ap.add_argument("--adjoint-tolerance", type=float, default=None,
+                    help="adjoint report: pressure-solve tolerance of the forward (default: the workload's 1e-5)")
+    ap.add_argument("--adjoint-horizons", default="10,20",
+                    help="adjoint report: shorter horizons reported beside STEPS (comma-separated)")
     ap.add_argument("--view-report", action="store_true",
                     help="instead of timing the step, time the live-buffer hand-off and the vec4 particle pack")
     ap.add_argument("--default-scene-report", action="store_true",
@@ -617,8 +623,10 @@ def main():
         return
 
     if args.adjoint_report:
-        print(json.dumps(adjoint_report(fsm, scenes, gpu_params, steps=args.adjoint_report, fp64=not args.no_fp64)),
-              flush=True)
+        hz = sorted({int(x) for x in args.adjoint_horizons.split(",") if 0 < int(x) <= args.adjoint_report}
+                    | {args.adjoint_report})
+        print(json.dumps(adjoint_report(fsm, scenes, gpu_params, horizons=hz, fp64=not args.no_fp64,
+                                        tolerance=args.adjoint_tolerance)), flush=True)
         return
 
     if args.scene_report:

@@ -146,6 +146,25 @@ int orc_run_f64(const orc_params *P, uint32_t n, double *px, double *py, double 
  * (pos/vel/pressure end in the final state) and returns a tape;
  * adj_backward turns dL/d(final state) in gpos/gvel into dL/d(initial state)
  * and fills grav with dL/d(gravity). */
+/* Decisions of the smooth forward (flip_oracle_adjoint.h), `steps` substeps
+ * of n particles on C cells: perm steps x n, occ and ct steps x C, cell
+ * steps x 3n (floor of each grid coordinate), bits steps x n (bit c: the
+ * coordinate less one half floors one lower; bit 3: CFL cap; bit 4 + 2a /
+ * 5 + 2a: clamped to the low / high wall on axis a). */
+typedef struct orc_smooth_rec {
+    uint32_t *perm, *occ, *ct;
+    int32_t *cell;
+    uint16_t *bits;
+    uint8_t *support; /* steps x (Fu + Fv + Fw): the face had a positive weight sum */
+} orc_smooth_rec;
+void orc_smooth_forward_f64(const orc_params *P, uint32_t n, double *const pos[3], double *const vel[3],
+                            double *pressure, uint32_t steps, const orc_smooth_rec *rec, int mode);
+void orc_smooth_record_states_f64(const orc_params *P, uint32_t n, uint32_t steps, const double *states,
+                                  const orc_smooth_rec *rec);
+void orc_smooth_forward_f32(const orc_params *P, uint32_t n, float *const pos[3], float *const vel[3],
+                            float *pressure, uint32_t steps, const orc_smooth_rec *rec, int mode);
+void orc_smooth_record_states_f32(const orc_params *P, uint32_t n, uint32_t steps, const float *states,
+                                  const orc_smooth_rec *rec);
 void *orc_adj_forward_f32(const orc_params *P, uint32_t n, float *const pos[3], float *const vel[3],
                           float *pressure, uint32_t steps, uint32_t every, uint32_t *rec_ct, uint32_t *rec_occ);
 int orc_adj_backward_f32(void *tape, float *const gpos[3], float *const gvel[3], double grav[3]);

@@ -13,7 +13,10 @@
  *     filter of the G2P gather, the CFL branch and the wall-clamp branch of
  *     the forward are held fixed;
  *   - fixed-point rounding of the P2G terms is the identity (num = sum w v,
- *     den = sum w);
+ *     den = sum w): the reverse pass replays each substep with P2G at
+ *     2^22 (ADJ_FIXED_SCALE; finer scales are kept) instead of the step's
+ *     2^16, so that rounding moves the linearization by ~4e-5 relative
+ *     instead of ~3e-3;
  *   - the pressure solve is exact: b_bar = A^-1 p_bar, solved here by CG in R
  *     to a relative residual of 1e-13 (double) / 1e-6 (float);
  *   - wall_friction is 0.
@@ -53,6 +56,7 @@ typedef struct {
     real *tmp, *div;
     ACC *num[3], *den[3];
     real *vel[3], *saved[3];
+    uint32_t fc_alloc[3];
 } adj_ws;
 
 static void adj_ws_alloc(const adj_tape *T, adj_ws *w) {
@@ -64,6 +68,7 @@ static void adj_ws_alloc(const adj_tape *T, adj_ws *w) {
     w->tmp = (real *)malloc(sizeof(real) * n);
     w->div = (real *)malloc(sizeof(real) * C);
     for (int a = 0; a < 3; ++a) {
+        w->fc_alloc[a] = T->fc[a];
         w->num[a] = (ACC *)malloc(sizeof(ACC) * T->fc[a]);
         w->den[a] = (ACC *)malloc(sizeof(ACC) * T->fc[a]);
         w->vel[a] = (real *)malloc(sizeof(real) * T->fc[a]);
@@ -431,9 +436,13 @@ void *FN(adj_tape_from_states)(const orc_params *P, uint32_t n, uint32_t steps, 
 
 /* Reverse pass: gpos/gvel in, dL/d(final state) in its order; out,
  * dL/d(initial state) in the initial order.  grav[3] receives dL/d(gravity). */
+#define ADJ_FIXED_SCALE 4194304.0f /* 2^22 */
+
 int FN(adj_backward)(void *tape, real *const gpos[3], real *const gvel[3], double grav[3]) {
     adj_tape *T = (adj_tape *)tape;
     const orc_params *P = &T->P;
+    orc_params Pf = *P; /* the reverse pass's replay: P2G at the finer fixed point */
+    if (Pf.fixed_scale < ADJ_FIXED_SCALE) Pf.fixed_scale = ADJ_FIXED_SCALE;
     const uint32_t n = T->n;
     real *pos[3], *vel[3], *pr = (real *)malloc(sizeof(real) * T->C);
     for (int a = 0; a < 3; ++a) {
@@ -456,8 +465,8 @@ int FN(adj_backward)(void *tape, real *const gpos[3], real *const gvel[3], doubl
         }
         for (int s = (int)s1 - 1; s >= (int)s0; --s) {
             adj_load(T, T->itape + (size_t)((uint32_t)s - s0) * adj_snap_len(T), pos, vel, pr);
-            adj_prefix(P, n, pos, vel, pr, &w);
-            adj_step(P, T, pos, vel, &w, gpos, gvel, grav);
+            adj_prefix(&Pf, n, pos, vel, pr, &w);
+            adj_step(&Pf, T, pos, vel, &w, gpos, gvel, grav);
         }
     }
     adj_ws_free(&w);
@@ -474,4 +483,226 @@ void FN(adj_free)(void *tape) {
     free(T->ckpt);
     free(T->itape);
     free(T);
+}
+
+/* ---- the smooth forward that the reverse pass differentiates ----
+ * A substep with every discrete decision taken from a record: the sort
+ * permutation, occupancies and cell types, each particle's stencil bases
+ * (floor of each grid coordinate, and whether the coordinate less one half
+ * floors one lower), its CFL branch and, per axis, its wall-clamp branch
+ * (which wall).  P2G sums real weights (no fixed-point rounding); the tent
+ * weights are those of the recorded bases (linear extrapolation where the
+ * particle has moved out of its recorded cell, every in-range tap kept);
+ * the pressure solve runs to a 1e-12 relative residual.  In record mode the
+ * substep takes its decisions itself and writes them to the record.  This is
+ * the function of the adjoint's conventions (flip_oracle_adjoint.h header):
+ * central differences of it must match the reverse pass. */
+static orc_stats CAT(g_smooth_cg_, SFX)[1024]; /* the pressure solve of substep s (mod 1024) */
+void FN(smooth_cg_stats)(uint32_t s, orc_stats *out) { *out = CAT(g_smooth_cg_, SFX)[s & 1023u]; }
+
+#define SMOOTH_CAPPED 8u
+#define SMOOTH_LO(a) (16u << (2 * (a)))
+#define SMOOTH_HI(a) (32u << (2 * (a)))
+
+static void smooth_substep(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pr,
+                           adj_ws *w, const orc_smooth_rec *rec, uint32_t s, int mode) {
+    /* mode bit 0: record the decisions; bit 1: replay with the stencils of
+     * the current positions (true tent weights) instead of the recorded ones */
+    const int record = mode & 1, free_stencils = !record && (mode & 2);
+    const uint32_t C = P->nx * P->ny * P->nz;
+    uint32_t *perm = rec->perm + (size_t)s * n, *occ = rec->occ + (size_t)s * C, *ct = rec->ct + (size_t)s * C;
+    int32_t *cell = rec->cell + (size_t)s * 3 * n;
+    uint16_t *bits = rec->bits + (size_t)s * n;
+    const real inv_h = RINVH(P);
+    if (record) {
+        FN(histogram)(P, n, pos[0], pos[1], pos[2], occ);
+        for (uint32_t p = 0; p < n; ++p) w->keys[p] = cell_key(P, pos[0][p], pos[1][p], pos[2][p]);
+        orc_stable_permutation(n, C, w->keys, perm);
+        classify(P, occ, ct);
+    }
+    for (int a = 0; a < 3; ++a) {
+        permute(n, perm, pos[a], w->tmp);
+        permute(n, perm, vel[a], w->tmp);
+    }
+    if (record) {
+#pragma omp parallel for schedule(static)
+        for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+            uint16_t b = 0;
+            for (int c = 0; c < 3; ++c) {
+                const real q = (pos[c][pp] - (real)P->origin[c]) * inv_h;
+                const int64_t f0 = sat_i32(RFLOOR(q)), f5 = sat_i32(RFLOOR(q + (real)-0.5));
+                cell[3 * pp + c] = (int32_t)f0;
+                if (f5 != f0) b |= (uint16_t)(1u << c);
+            }
+            bits[pp] = b;
+        }
+    }
+    /* P2G with real sums over the recorded stencils. */
+    real *ns[3], *ds[3];
+    for (int a = 0; a < 3; ++a) {
+        ns[a] = (real *)calloc(w->fc_alloc[a], sizeof(real));
+        ds[a] = (real *)calloc(w->fc_alloc[a], sizeof(real));
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            int64_t base[3];
+            real t3[3];
+            for (int c = 0; c < 3; ++c) {
+                const real g = (pos[c][pp] - (real)P->origin[c]) * inv_h + k_off[a][c];
+                base[c] = free_stencils ? sat_i32(RFLOOR(g))
+                                        : cell[3 * pp + c] - (c != a && (bits[pp] >> c & 1u) ? 1 : 0);
+                t3[c] = g - (real)base[c];
+            }
+            for (int dk = 0; dk < 2; ++dk)
+                for (int dj = 0; dj < 2; ++dj)
+                    for (int di = 0; di < 2; ++di) {
+                        const int64_t ii = base[0] + di, jj = base[1] + dj, kk = base[2] + dk;
+                        if (ii < 0 || ii >= (int64_t)dim[0] || jj < 0 || jj >= (int64_t)dim[1] || kk < 0 ||
+                            kk >= (int64_t)dim[2])
+                            continue;
+                        const real wgt = (di ? t3[0] : (real)1.0 - t3[0]) * (dj ? t3[1] : (real)1.0 - t3[1]) *
+                                         (dk ? t3[2] : (real)1.0 - t3[2]);
+                        const uint64_t idx = (uint64_t)ii + dim[0] * ((uint64_t)jj + dim[1] * (uint64_t)kk);
+                        const real add = wgt * vel[a][pp];
+#pragma omp atomic
+                        ns[a][idx] += add;
+#pragma omp atomic
+                        ds[a][idx] += wgt;
+                    }
+        }
+    }
+    /* Faces with weight in the record keep num/den (a weighted mean of their
+     * particles' velocities) and the others stay 0, so that no face jumps
+     * between 0 and its first particle's velocity. */
+    uint8_t *sup = rec->support + (size_t)s * (w->fc_alloc[0] + w->fc_alloc[1] + w->fc_alloc[2]);
+    for (int a = 0; a < 3; ++a) {
+        for (uint32_t f = 0; f < w->fc_alloc[a]; ++f) {
+            if (record) sup[f] = ds[a][f] > (real)0.0;
+            w->vel[a][f] = sup[f] && ds[a][f] != (real)0.0 ? ns[a][f] / ds[a][f] : (real)0.0;
+            w->saved[a][f] = w->vel[a][f];
+        }
+        sup += w->fc_alloc[a];
+        free(ns[a]);
+        free(ds[a]);
+    }
+    forces_enforce(P, ct, w->vel);
+    divergence(P, w->vel, ct, occ, w->div);
+    orc_params Pe = *P;
+    Pe.tol = 1e-12f;
+    Pe.tol_sq = 1e-24f;
+    Pe.pressure_iters = 50000;
+    Pe.reduce_mode = 1;
+    orc_stats st;
+    memset(&st, 0, sizeof st);
+    FN(pressure_solve)(&Pe, ct, w->div, pr, &st);
+    CAT(g_smooth_cg_, SFX)[s & 1023u] = st;
+    gradient_enforce(P, pr, ct, w->vel);
+    /* G2P over the recorded stencils with the recorded branches. */
+    const real h = (real)P->h, dt = (real)P->dt, alpha = (real)P->flip_blend;
+    const real maxs = RMAX((real)P->cfl, (real)1.0) * h / dt, hband = h * (real)1.05f;
+    real lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = (real)P->origin[c] + hband;
+        hi[c] = (real)P->origin[c] + (real)(float)(c == 0 ? P->nx : (c == 1 ? P->ny : P->nz)) * h - hband;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+        const real Pp[3] = {pos[0][pp], pos[1][pp], pos[2][pp]};
+        real pic[3], sv[3];
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            int64_t base[3];
+            real t3[3];
+            for (int c = 0; c < 3; ++c) {
+                const real g = (Pp[c] - (real)P->origin[c]) * inv_h + k_off[a][c];
+                base[c] = free_stencils ? sat_i32(RFLOOR(g))
+                                        : cell[3 * pp + c] - (c != a && (bits[pp] >> c & 1u) ? 1 : 0);
+                t3[c] = g - (real)base[c];
+            }
+            real fin = 0, sav = 0, W = 0;
+            for (int dk = 0; dk < 2; ++dk)
+                for (int dj = 0; dj < 2; ++dj)
+                    for (int di = 0; di < 2; ++di) {
+                        const int64_t ii = base[0] + di, jj = base[1] + dj, kk = base[2] + dk;
+                        if (ii < 0 || ii >= (int64_t)dim[0] || jj < 0 || jj >= (int64_t)dim[1] || kk < 0 ||
+                            kk >= (int64_t)dim[2])
+                            continue;
+                        if (touches_static_solid(P, a, ii, jj, kk)) continue;
+                        const real wgt = (di ? t3[0] : (real)1.0 - t3[0]) * (dj ? t3[1] : (real)1.0 - t3[1]) *
+                                         (dk ? t3[2] : (real)1.0 - t3[2]);
+                        const uint64_t idx = (uint64_t)ii + dim[0] * ((uint64_t)jj + dim[1] * (uint64_t)kk);
+                        fin += wgt * w->vel[a][idx];
+                        sav += wgt * w->saved[a][idx];
+                        W += wgt;
+                    }
+            pic[a] = W > (real)0.0 ? fin / W : (real)0.0;
+            sv[a] = W > (real)0.0 ? sav / W : (real)0.0;
+        }
+        real nv[3];
+        for (int a = 0; a < 3; ++a) nv[a] = pic[a] * ((real)1.0 - alpha) + (vel[a][pp] + (pic[a] - sv[a])) * alpha;
+        const real spd = RSQRT(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        uint16_t b = bits[pp];
+        if (record && spd > maxs && spd > (real)0.0) b |= SMOOTH_CAPPED;
+        if ((b & SMOOTH_CAPPED) && spd > (real)0.0)
+            for (int a = 0; a < 3; ++a) nv[a] = nv[a] * (maxs / spd);
+        for (int a = 0; a < 3; ++a) {
+            real np = Pp[a] + nv[a] * dt;
+            if (record) {
+                if (np < lo[a]) b |= (uint16_t)SMOOTH_LO(a);
+                else if (np > hi[a]) b |= (uint16_t)SMOOTH_HI(a);
+            }
+            if (b & SMOOTH_LO(a)) {
+                np = lo[a];
+                nv[a] = (real)0.0;
+            } else if (b & SMOOTH_HI(a)) {
+                np = hi[a];
+                nv[a] = (real)0.0;
+            }
+            pos[a][pp] = np;
+            vel[a][pp] = nv[a];
+        }
+        if (record) bits[pp] = b;
+    }
+}
+
+/* `steps` smooth substeps from (pos, vel, pressure), recording the
+ * decisions (record = 1) or replaying them (record = 0). */
+void FN(smooth_forward)(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pressure,
+                        uint32_t steps, const orc_smooth_rec *rec, int mode) {
+    adj_tape *T = adj_tape_new(P, n, 1, 1);
+    adj_ws w;
+    adj_ws_alloc(T, &w);
+    for (uint32_t s = 0; s < steps; ++s) smooth_substep(P, n, pos, vel, pressure, &w, rec, s, mode);
+    adj_ws_free(&w);
+    FN(adj_free)(T);
+}
+
+/* The decisions of a trajectory computed elsewhere: substep s is taken from
+ * the given input state s (states: steps x (pos xyz, vel xyz, pressure), the
+ * layout of FN(adj_tape_from_states)) in record mode, its result discarded. */
+void FN(smooth_record_states)(const orc_params *P, uint32_t n, uint32_t steps, const real *states,
+                              const orc_smooth_rec *rec) {
+    adj_tape *T = adj_tape_new(P, n, 1, 1);
+    adj_ws w;
+    adj_ws_alloc(T, &w);
+    real *pos[3], *vel[3], *pr = (real *)malloc(sizeof(real) * T->C);
+    for (int a = 0; a < 3; ++a) {
+        pos[a] = (real *)malloc(sizeof(real) * (n ? n : 1));
+        vel[a] = (real *)malloc(sizeof(real) * (n ? n : 1));
+    }
+    for (uint32_t s = 0; s < steps; ++s) {
+        adj_load(T, states + (size_t)s * adj_snap_len(T), pos, vel, pr);
+        smooth_substep(P, n, pos, vel, pr, &w, rec, s, 1);
+    }
+    for (int a = 0; a < 3; ++a) {
+        free(pos[a]);
+        free(vel[a]);
+    }
+    free(pr);
+    adj_ws_free(&w);
+    FN(adj_free)(T);
 }

@@ -169,9 +169,11 @@ static void p2g(const orc_params *P, uint32_t n, const real *px, const real *py,
  * axis, in that order (src/gpu/fluid.rs:1444-1460). */
 static void enforce_axis(const orc_params *P, int a, const uint32_t *ct, real *vel);
 
+/* forces and the pre-pressure enforce, after normalize and save. */
+static void forces_enforce(const orc_params *P, const uint32_t *ct, real *vel[3]);
+
 static void p2g_epilogue(const orc_params *P, const uint32_t *ct, ACC *num[3], ACC *den[3],
                          real *vel[3], real *saved[3]) {
-    const uint32_t nx = P->nx, ny = P->ny;
     for (int a = 0; a < 3; ++a) {
         uint32_t d[3];
         face_dims(P, a, d);
@@ -183,6 +185,11 @@ static void p2g_epilogue(const orc_params *P, const uint32_t *ct, ACC *num[3], A
         }
         memcpy(saved[a], vel[a], sizeof(real) * (size_t)cnt);
     }
+    forces_enforce(P, ct, vel);
+}
+
+static void forces_enforce(const orc_params *P, const uint32_t *ct, real *vel[3]) {
+    const uint32_t nx = P->nx, ny = P->ny;
     for (int a = 0; a < 3; ++a) {
         uint32_t d[3];
         face_dims(P, a, d);

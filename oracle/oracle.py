@@ -437,6 +437,61 @@ def run_frozen(P, pos, vel, ct_steps, occ_steps, pressure=None, f64=True):
     return np.stack(cols_p, 1), np.stack(cols_v, 1), pr
 
 
+class _SmoothRecC(C.Structure):
+    _fields_ = [("perm", C.c_void_p), ("occ", C.c_void_p), ("ct", C.c_void_p), ("cell", C.c_void_p),
+                ("bits", C.c_void_p), ("support", C.c_void_p)]
+
+
+class SmoothRecord:
+    """The decisions of `steps` substeps of the smooth forward
+    (orc_smooth_rec): sort permutation, occupancies, cell types, stencil
+    bases and the CFL / wall-clamp branches of every particle."""
+
+    def __init__(self, P, n, steps):
+        C_ = P.nx * P.ny * P.nz
+        self.n, self.steps = n, steps
+        self.perm = np.zeros((steps, max(n, 1)), np.uint32)
+        self.occ = np.zeros((steps, C_), np.uint32)
+        self.ct = np.zeros((steps, C_), np.uint32)
+        self.cell = np.zeros((steps, 3 * max(n, 1)), np.int32)
+        self.bits = np.zeros((steps, max(n, 1)), np.uint16)
+        self.support = np.zeros((steps, sum(face_count(P, a) for a in range(3))), np.uint8)
+        self._c = _SmoothRecC(*[a.ctypes.data for a in (self.perm, self.occ, self.ct, self.cell, self.bits,
+                                                        self.support)])
+
+
+def smooth_forward(P, pos, vel, rec, pressure=None, record=False, free_stencils=False, f64=True):
+    """orc_smooth_forward_*: rec.steps substeps of the smooth forward from
+    (pos, vel, pressure), taking every discrete decision from `rec` (or, with
+    record=True, taking them itself and writing them to `rec`).  With
+    free_stencils the replay takes each particle's stencil from its current
+    position (the true tent weights) instead of the record.  Returns the
+    final (pos, vel, pressure), particles in the last sort's order."""
+    real = np.float64 if f64 else np.float32
+    cols_p = [np.array(pos[:, a], dtype=real, copy=True) for a in range(3)]
+    cols_v = [np.array(vel[:, a], dtype=real, copy=True) for a in range(3)]
+    pr = np.zeros(P.nx * P.ny * P.nz, real) if pressure is None else np.array(pressure, real, copy=True)
+    vp = C.c_void_p * 3
+    getattr(lib(), "orc_smooth_forward_f64" if f64 else "orc_smooth_forward_f32")(
+        C.byref(P), C.c_uint32(len(pos)), vp(*[ptr(c) for c in cols_p]), vp(*[ptr(c) for c in cols_v]), ptr(pr),
+        C.c_uint32(rec.steps), C.byref(rec._c), C.c_int(int(record) | (2 if free_stencils else 0)))
+    return np.stack(cols_p, 1), np.stack(cols_v, 1), pr
+
+
+def smooth_record_states(P, states, f64=True):
+    """The smooth forward's decisions along a trajectory computed elsewhere:
+    substep s taken (in record mode) from states[s] = (pos, vel, pressure) in
+    that substep's input order.  Returns a SmoothRecord."""
+    real = np.float64 if f64 else np.float32
+    n = len(states[0][0])
+    rec = SmoothRecord(P, n, len(states))
+    flat = np.concatenate([np.concatenate([np.asarray(p, real).T.ravel(), np.asarray(v, real).T.ravel(),
+                                           np.asarray(pr, real).ravel()]) for p, v, pr in states])
+    getattr(lib(), "orc_smooth_record_states_f64" if f64 else "orc_smooth_record_states_f32")(
+        C.byref(P), C.c_uint32(n), C.c_uint32(len(states)), ptr(flat), C.byref(rec._c))
+    return rec
+
+
 # ---------------------------------------------------------------------------
 # Scene presets, spawn counts and the initial lattice (flip_oracle_scene.c).
 
