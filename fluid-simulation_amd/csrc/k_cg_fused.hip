@@ -51,7 +51,11 @@ constexpr size_t kSyncBarCount = 0, kSyncGroups = 32, kSyncAbort = 32 + 8 * 32, 
                  kSyncSlots = kSyncGen + 32;
 }  // namespace
 
-size_t cg_sync_words(int max_blocks) { return kSyncSlots + 16u * (size_t)(max_blocks > 0 ? max_blocks : 1); }
+// Two slot areas (exchange, data barrier) of two epoch halves of max_blocks
+// slots of kSlotStride u64 each.
+size_t cg_sync_words(int max_blocks) {
+    return kSyncSlots + 8u * kSlotStride * (size_t)(max_blocks > 0 ? max_blocks : 1);
+}
 
 void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks) {
     w.sync = area;
@@ -61,7 +65,8 @@ void cg_bind_sync(CgWork& w, uint32_t* area, int max_blocks) {
     w.abort_flag = area + kSyncAbort;
     w.bar_gen = area + kSyncGen;
     w.slots = reinterpret_cast<uint64_t*>(area + kSyncSlots);
-    w.slots_data = reinterpret_cast<uint64_t*>(area + kSyncSlots + 8u * (size_t)(max_blocks > 0 ? max_blocks : 1));
+    w.slots_data =
+        reinterpret_cast<uint64_t*>(area + kSyncSlots + 4u * kSlotStride * (size_t)(max_blocks > 0 ? max_blocks : 1));
     w.spin_ticks = kSpinTimeoutTicks;
 }
 
@@ -222,20 +227,20 @@ template <int THREADS>
 __device__ __forceinline__ bool grid_exchange_all(double v, const CgWork& w, uint64_t* slots, uint32_t G, uint32_t epoch,
                                   double& out) {
     __shared__ int s_abort;
-    uint64_t* half = slots + (size_t)(epoch & 1u) * G * 2u;
+    uint64_t* half = slots + (size_t)(epoch & 1u) * G * kSlotStride;
     const double in[1] = {v};
     double bs[1];
     block_sum_nv<THREADS, 1>(in, bs);
     if (threadIdx.x == 0) {
         s_abort = 0;
-        publish_tagged(half, blockIdx.x, epoch, bs[0], 0u);
+        publish_tagged(half, blockIdx.x, epoch, bs[0], 0u, kSlotStride);
     }
     __syncthreads();
     double acc[1] = {0.0};
     for (uint32_t i = threadIdx.x; i < G; i += THREADS) {
         double x;
         uint32_t f;
-        if (!poll_tagged(half, i, epoch, w.abort_flag, x, f, w.spin_ticks)) {
+        if (!poll_tagged(half, i, epoch, w.abort_flag, x, f, w.spin_ticks, kSlotStride)) {
             s_abort = 1;
             break;
         }
@@ -268,14 +273,14 @@ __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* 
                                            double& out) {
     __shared__ double s_out;
     __shared__ int s_abort;
-    uint64_t* half = slots + (size_t)(epoch & 1u) * G * 2u;
+    uint64_t* half = slots + (size_t)(epoch & 1u) * G * kSlotStride;
     const double in[1] = {v};
     double bs[1];
     block_sum_nv<THREADS, 1>(in, bs);
     if (threadIdx.x == 0) {
         s_abort = 0;
         if (FENCED) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        publish_tagged(half, blockIdx.x, epoch, bs[0], 0u);
+        publish_tagged(half, blockIdx.x, epoch, bs[0], 0u, kSlotStride);
     }
     if (threadIdx.x < 64) {
         const uint32_t lane = threadIdx.x;
@@ -298,8 +303,8 @@ __device__ __forceinline__ bool tagged_sum(double v, const CgWork& w, uint64_t* 
                 hi[k] = lo[k] = 0;
                 if (64u * k < G) {
                     const uint32_t i = lane + 64u * k < G ? lane + 64u * k : G - 1u;
-                    hi[k] = __hip_atomic_load(half + 2u * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    lo[k] = __hip_atomic_load(half + 2u * i + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    hi[k] = __hip_atomic_load(half + kSlotStride * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lo[k] = __hip_atomic_load(half + kSlotStride * i + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
 #pragma unroll
