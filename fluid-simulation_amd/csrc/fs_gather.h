@@ -33,6 +33,7 @@ constexpr int kGThreads = 512;
 constexpr int kGThreadsSmall = 1024;
 constexpr int kGThreadsLarge = 256;
 constexpr uint32_t kGSmallGrid = 1024;
+constexpr int kGBatch = 4;  // particles whose loads a thread issues together
 
 // Brick shapes of the gather menu: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells
 // (0 measured fastest at 128^3, 256^3 and on the 64-tank batch).
@@ -169,20 +170,51 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
         }
     };
 
+    // Items are taken kGBatch at a time per thread: the permutation entries,
+    // then the positions and velocities of all of them are loaded before any
+    // is evaluated, so a thread keeps kGBatch particles' loads in flight
+    // instead of waiting out one particle's load chain at a time.
     int row = 0;
-    for (uint32_t base = 0; base < total; base += THREADS) {
-        // Each wave takes every (THREADS / 64)-th item of the chunk, so its lanes fall in
-        // different cells (sorted particles of one cell share faces, and
-        // same-address LDS atomics within a wave serialize).
-        const uint32_t t = base + (threadIdx.x & 63u) * (THREADS / 64) + (threadIdx.x >> 6);
-        if (t >= total) continue;
-        // Find the row holding flattened item t (rows are few; linear advance).
-        while (s_row_pref[row + 1] <= t) ++row;
-        const uint32_t d = s_row_start[row] + (t - s_row_pref[row]);
-        const uint32_t p = perm ? perm[d] : d;
-        const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
-                             load_vel<BF16>(sp.vel[2], p)};
-        add_particle(sp.pos[0][p], sp.pos[1][p], sp.pos[2][p], pv);
+    for (uint32_t base = 0; base < total; base += THREADS * kGBatch) {
+        uint32_t pk[kGBatch];
+        bool live[kGBatch];
+#pragma unroll
+        for (int k = 0; k < kGBatch; ++k) {
+            // Each wave takes every (THREADS / 64)-th item of its chunk, so its lanes fall in
+            // different cells (sorted particles of one cell share faces, and
+            // same-address LDS atomics within a wave serialize).
+            const uint32_t t = base + (uint32_t)k * THREADS + (threadIdx.x & 63u) * (THREADS / 64) + (threadIdx.x >> 6);
+            live[k] = t < total;
+            pk[k] = 0;
+            if (live[k]) {
+                // The row holding flattened item t (t grows with k; linear advance).
+                while (s_row_pref[row + 1] <= t) ++row;
+                pk[k] = s_row_start[row] + (t - s_row_pref[row]);
+            }
+        }
+        if (perm) {
+#pragma unroll
+            for (int k = 0; k < kGBatch; ++k)
+                if (live[k]) pk[k] = perm[pk[k]];
+        }
+        float px[kGBatch], py[kGBatch], pz[kGBatch], vx[kGBatch], vy[kGBatch], vz[kGBatch];
+#pragma unroll
+        for (int k = 0; k < kGBatch; ++k) {
+            if (!live[k]) continue;
+            const uint32_t p = pk[k];
+            px[k] = sp.pos[0][p];
+            py[k] = sp.pos[1][p];
+            pz[k] = sp.pos[2][p];
+            vx[k] = load_vel<BF16>(sp.vel[0], p);
+            vy[k] = load_vel<BF16>(sp.vel[1], p);
+            vz[k] = load_vel<BF16>(sp.vel[2], p);
+        }
+#pragma unroll
+        for (int k = 0; k < kGBatch; ++k) {
+            if (!live[k]) continue;
+            const float pv[3] = {vx[k], vy[k], vz[k]};
+            add_particle(px[k], py[k], pz[k], pv);
+        }
     }
     for (uint32_t e = threadIdx.x; e < mis_n; e += THREADS) {
         const uint32_t p = mis_list[e];
