@@ -221,8 +221,19 @@ def adjoint_report(fsm, scenes, gpu_params, horizons=(10, 20, 60), every=10, hs=
                 occupancies, face supports, CFL and wall-clamp branches; stencils
                 from the current positions), real P2G sums and an exact pressure
                 solve: the function the adjoint's conventions define.
+    tlm:        forward-mode derivatives (dual numbers, oracle.tlm_from_states)
+                at each input state of the GPU's own trajectory, carried
+                through all H substeps, with no step size, so they exist at
+                any horizon (central differences stop converging once the run
+                is chaotic).  "replay": linearized where the reverse pass
+                linearizes (P2G at 2^22, the substep's CG tolerance), the
+                other product of the same Jacobian; "smooth": linearized at
+                the smooth substep (real sums, exact solve), which shows how
+                far the derivative moves with the linearization point.
     plain_fd:   central differences (h = 1e-3) of the GPU forward itself, whose
-                discrete decisions move with the perturbation."""
+                discrete decisions move with the perturbation.
+    Gravity is a float32 parameter: every difference in gravity divides by
+    the step that the rounded values g0 +- h actually take."""
     import oracle
 
     import helpers
@@ -284,9 +295,14 @@ def adjoint_report(fsm, scenes, gpu_params, horizons=(10, 20, 60), every=10, hs=
             tt.close()
             return loss(pp)
 
+        def g_step(h):
+            gp, gm = float(np.float32(g0 + h)), float(np.float32(g0 - h))
+            return gp, gm, gp - gm
+
         hp = 1e-3
         fd = [(L_gpu(v0 + hp * np.eye(3)[a], g0) - L_gpu(v0 - hp * np.eye(3)[a], g0)) / (2 * hp) for a in range(3)]
-        fd.append((L_gpu(v0, g0 + hp) - L_gpu(v0, g0 - hp)) / (2 * hp))
+        gp, gm, gs = g_step(hp)
+        fd.append((L_gpu(v0, gp) - L_gpu(v0, gm)) / gs)
         ent["plain_fd"] = {"h": hp, "grad": fd, "rel_err_vs_gpu": rel(g_gpu, fd)}
         if fp64:
             # The GPU trajectory: every substep's input state (read-out order).
@@ -305,6 +321,26 @@ def adjoint_report(fsm, scenes, gpu_params, horizons=(10, 20, 60), every=10, hs=
             g64 = grad4(ogv, ogg)
             ent["fp64"] = {"grad": g64, "seconds": time.perf_counter() - t0, "rel_err_gpu_vs_fp64": rel(g_gpu, g64),
                            "rel_l2_per_particle_vel_grad": rel(gv, ogv)}
+            t0 = time.perf_counter()
+            n = len(pf)
+            dpos0 = np.zeros((4, n, 3))
+            dvel0 = np.zeros((4, n, 3))
+            for a in range(3):
+                dvel0[a, :, a] = 1.0
+            dgrav = np.zeros((4, 3))
+            dgrav[3, 1] = 1.0
+            up = upstream(pf)
+            ent["tlm"] = {}
+            for name, replay in (("replay", True), ("smooth", False)):
+                t0 = time.perf_counter()
+                tp, _ = oracle.tlm_from_states(P64, states, dpos0, dvel0, dgrav, replay=replay)
+                g_tlm = [float((up * tp[k]).sum()) for k in range(4)]
+                del tp
+                ent["tlm"][name] = {"grad": g_tlm, "seconds": time.perf_counter() - t0,
+                                    "rel_err_gpu_vs_tlm": rel(g_gpu, g_tlm), "rel_err_fp64_vs_tlm": rel(g64, g_tlm)}
+                print(f"adjoint report: H={H} forward mode ({name}) {g_tlm} rel err gpu "
+                      f"{ent['tlm'][name]['rel_err_gpu_vs_tlm']:.3e}", file=sys.stderr, flush=True)
+            del dpos0, dvel0
             rec = oracle.smooth_record_states(P64, states)
             del states
             x0 = sc["pos"].astype(np.float64)
@@ -323,7 +359,8 @@ def adjoint_report(fsm, scenes, gpu_params, horizons=(10, 20, 60), every=10, hs=
                 t0 = time.perf_counter()
                 fd = [(L_frozen(v0 + h * np.eye(3)[a], g0) - L_frozen(v0 - h * np.eye(3)[a], g0)) / (2 * h)
                       for a in range(3)]
-                fd.append((L_frozen(v0, g0 + h) - L_frozen(v0, g0 - h)) / (2 * h))
+                gp, gm, gs = g_step(h)
+                fd.append((L_frozen(v0, gp) - L_frozen(v0, gm)) / gs)
                 ent["frozen_fd"][f"{h:g}"] = {"grad": fd, "rel_err_gpu_vs_fd": rel(g_gpu, fd),
                                               "rel_err_fp64_vs_fd": rel(g64, fd), "seconds": time.perf_counter() - t0}
                 print(f"adjoint report: H={H} frozen fd h={h:g} {fd}", file=sys.stderr, flush=True)

@@ -165,6 +165,14 @@ void orc_smooth_forward_f32(const orc_params *P, uint32_t n, float *const pos[3]
                             float *pressure, uint32_t steps, const orc_smooth_rec *rec, int mode);
 void orc_smooth_record_states_f32(const orc_params *P, uint32_t n, uint32_t steps, const float *states,
                                   const orc_smooth_rec *rec);
+/* Forward-mode (tangent-linear) derivatives of the smooth forward along given
+ * input states (flip_oracle_adjoint.h): K directions, dpos/dvel K x 3 arrays
+ * of n each (updated in place), dgrav 3 K; replay 0 linearizes at the smooth
+ * substep, 1 at the reverse pass's replay of it. */
+void orc_tlm_from_states_f64(const orc_params *P, uint32_t n, uint32_t steps, const double *states, uint32_t K,
+                             double *const *dpos, double *const *dvel, const float *dgrav, int replay);
+void orc_tlm_from_states_f32(const orc_params *P, uint32_t n, uint32_t steps, const float *states, uint32_t K,
+                             float *const *dpos, float *const *dvel, const float *dgrav, int replay);
 void *orc_adj_forward_f32(const orc_params *P, uint32_t n, float *const pos[3], float *const vel[3],
                           float *pressure, uint32_t steps, uint32_t every, uint32_t *rec_ct, uint32_t *rec_occ);
 int orc_adj_backward_f32(void *tape, float *const gpos[3], float *const gvel[3], double grav[3]);

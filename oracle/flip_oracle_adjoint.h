@@ -24,6 +24,8 @@
  * each interval recomputed during the reverse pass.
  */
 
+#define ADJ_FIXED_SCALE 4194304.0f /* 2^22: the fixed point of the reverse pass's P2G replay */
+
 typedef struct {
     orc_params P;
     uint32_t n, C, fc[3], steps, every, K;
@@ -436,8 +438,6 @@ void *FN(adj_tape_from_states)(const orc_params *P, uint32_t n, uint32_t steps, 
 
 /* Reverse pass: gpos/gvel in, dL/d(final state) in its order; out,
  * dL/d(initial state) in the initial order.  grav[3] receives dL/d(gravity). */
-#define ADJ_FIXED_SCALE 4194304.0f /* 2^22 */
-
 int FN(adj_backward)(void *tape, real *const gpos[3], real *const gvel[3], double grav[3]) {
     adj_tape *T = (adj_tape *)tape;
     const orc_params *P = &T->P;
@@ -697,6 +697,336 @@ void FN(smooth_record_states)(const orc_params *P, uint32_t n, uint32_t steps, c
     for (uint32_t s = 0; s < steps; ++s) {
         adj_load(T, states + (size_t)s * adj_snap_len(T), pos, vel, pr);
         smooth_substep(P, n, pos, vel, pr, &w, rec, s, 1);
+    }
+    for (int a = 0; a < 3; ++a) {
+        free(pos[a]);
+        free(vel[a]);
+    }
+    free(pr);
+    adj_ws_free(&w);
+    FN(adj_free)(T);
+}
+
+/* ---- tangent-linear (forward-mode) derivative of the smooth forward ----
+ * tlm_substep: one smooth substep from the state (pos, vel, pr), taking its
+ * decisions from that state (record mode of smooth_substep: sort order,
+ * occupancies, cell types, stencil bases, face supports, CFL and wall-clamp
+ * branches), together with K tangent directions: dpos[k][c] and dvel[k][c]
+ * (n each) and dgrav[3 k + c], the tangent of the gravity vector.  Every
+ * operation of smooth_substep is differentiated exactly (dual numbers), so
+ * the tangents leave the substep as J (dpos, dvel, dgrav) with J the
+ * Jacobian of the smooth substep at this state.  The pressure tangent solves
+ * A dp = -(rho h^2 / dt) ddiv with the substep's operator to a 1e-12 relative
+ * residual (the exact solve's result does not depend on its starting
+ * guess).  This is forward mode: it shares no code with the reverse pass
+ * above beyond the forward's own building blocks (forces_enforce,
+ * divergence, pressure_solve, gradient_enforce, which are linear in the
+ * tangent once the decisions are fixed).
+ * The Jacobian depends on the point it is taken at: the face values that
+ * G2P's weights multiply and the decisions taken from them.  replay = 0
+ * takes it at the smooth substep (real P2G sums, a 1e-12 pressure solve);
+ * replay = 1 at the state the reverse pass replays (P2G sums at the fixed
+ * point max(scale, ADJ_FIXED_SCALE) with their support den > 0 and the
+ * normalization's denominator den / scale, and the substep's own pressure
+ * solve: its tolerance, cap and warm start).  With replay = 1, forward mode
+ * and the reverse pass are the two products of one Jacobian, so
+ * <w, J u> = <J^T w, u> holds to rounding. */
+static void tlm_substep(const orc_params *P, uint32_t n, real *const pos[3], real *const vel[3], real *pr,
+                        adj_ws *w, uint32_t K, real *const *dpos, real *const *dvel, const float *dgrav,
+                        int replay) {
+    const real fscale = (real)(P->fixed_scale > ADJ_FIXED_SCALE ? P->fixed_scale : ADJ_FIXED_SCALE);
+    const uint32_t C = P->nx * P->ny * P->nz;
+    const real inv_h = RINVH(P);
+    FN(histogram)(P, n, pos[0], pos[1], pos[2], w->occ);
+    for (uint32_t p = 0; p < n; ++p) w->keys[p] = cell_key(P, pos[0][p], pos[1][p], pos[2][p]);
+    orc_stable_permutation(n, C, w->keys, w->perm);
+    classify(P, w->occ, w->ct);
+    for (int a = 0; a < 3; ++a) {
+        permute(n, w->perm, pos[a], w->tmp);
+        permute(n, w->perm, vel[a], w->tmp);
+        for (uint32_t k = 0; k < K; ++k) {
+            permute(n, w->perm, dpos[3 * k + a], w->tmp);
+            permute(n, w->perm, dvel[3 * k + a], w->tmp);
+        }
+    }
+    /* P2G: real sums and their tangents; then normalize (with support),
+     * save.  Face arrays per axis: ns, ds, and per direction dns, dds. */
+    real *ns[3], *ds[3], *dns[3], *dds[3], *dv[3], *dsv[3];
+    int64_t *nfx[3], *dfx[3]; /* replay = 1: the fixed-point sums */
+    for (int a = 0; a < 3; ++a) {
+        const size_t F = w->fc_alloc[a];
+        nfx[a] = replay ? (int64_t *)calloc(F, sizeof(int64_t)) : NULL;
+        dfx[a] = replay ? (int64_t *)calloc(F, sizeof(int64_t)) : NULL;
+        ns[a] = (real *)calloc(F, sizeof(real));
+        ds[a] = (real *)calloc(F, sizeof(real));
+        dns[a] = (real *)calloc(F * K, sizeof(real));
+        dds[a] = (real *)calloc(F * K, sizeof(real));
+        dv[a] = (real *)calloc(F * K, sizeof(real));
+        dsv[a] = (real *)calloc(F * K, sizeof(real));
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            int64_t base[3];
+            real t3[3];
+            for (int c = 0; c < 3; ++c) {
+                const real g = (pos[c][pp] - (real)P->origin[c]) * inv_h + k_off[a][c];
+                base[c] = sat_i32(RFLOOR(g));
+                t3[c] = g - (real)base[c];
+            }
+            for (int dk = 0; dk < 2; ++dk)
+                for (int dj = 0; dj < 2; ++dj)
+                    for (int di = 0; di < 2; ++di) {
+                        const int64_t ii = base[0] + di, jj = base[1] + dj, kk = base[2] + dk;
+                        if (ii < 0 || ii >= (int64_t)dim[0] || jj < 0 || jj >= (int64_t)dim[1] || kk < 0 ||
+                            kk >= (int64_t)dim[2])
+                            continue;
+                        const int sel[3] = {di, dj, dk};
+                        real f[3];
+                        for (int c = 0; c < 3; ++c) f[c] = sel[c] ? t3[c] : (real)1.0 - t3[c];
+                        const real wgt = f[0] * f[1] * f[2];
+                        const uint64_t idx = (uint64_t)ii + dim[0] * ((uint64_t)jj + dim[1] * (uint64_t)kk);
+                        const real add = wgt * vel[a][pp];
+#pragma omp atomic
+                        ns[a][idx] += add;
+#pragma omp atomic
+                        ds[a][idx] += wgt;
+                        if (replay && wgt > (real)0.0) {
+                            /* the per-tap terms of p2g_component */
+                            const int64_t fn = sat_i32(RRINT(wgt * vel[a][pp] * fscale));
+                            const int64_t fd = sat_i32(RRINT(wgt * fscale));
+#pragma omp atomic
+                            nfx[a][idx] += fn;
+#pragma omp atomic
+                            dfx[a][idx] += fd;
+                        }
+                        for (uint32_t k = 0; k < K; ++k) {
+                            /* d f_c = +-d t_c = +-dpos_c inv_h; d w = sum_c d f_c prod_{c' != c} f_c' */
+                            real dw = (real)0.0;
+                            for (int c = 0; c < 3; ++c) {
+                                const real dtc = dpos[3 * k + c][pp] * inv_h;
+                                dw += (sel[c] ? dtc : -dtc) * f[(c + 1) % 3] * f[(c + 2) % 3];
+                            }
+                            const real dadd = dw * vel[a][pp] + wgt * dvel[3 * k + a][pp];
+                            const size_t ki = (size_t)k * w->fc_alloc[a] + idx;
+#pragma omp atomic
+                            dns[a][ki] += dadd;
+#pragma omp atomic
+                            dds[a][ki] += dw;
+                        }
+                    }
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        const size_t F = w->fc_alloc[a];
+        for (size_t f = 0; f < F; ++f) {
+            const int sup = replay ? dfx[a][f] > 0 : ds[a][f] > (real)0.0;
+            const real den = replay ? (real)dfx[a][f] / fscale : ds[a][f];
+            if (replay)
+                w->vel[a][f] = sup ? (real)nfx[a][f] / (real)dfx[a][f] : (real)0.0;
+            else
+                w->vel[a][f] = sup ? ns[a][f] / ds[a][f] : (real)0.0;
+            w->saved[a][f] = w->vel[a][f];
+            for (uint32_t k = 0; k < K; ++k) {
+                const size_t ki = (size_t)k * F + f;
+                dv[a][ki] = sup ? (dns[a][ki] - w->vel[a][f] * dds[a][ki]) / den : (real)0.0;
+                dsv[a][ki] = dv[a][ki];
+            }
+        }
+        free(ns[a]);
+        free(ds[a]);
+        free(nfx[a]);
+        free(dfx[a]);
+        free(dns[a]);
+        free(dds[a]);
+    }
+    /* forces + enforce: affine in the face field, the gravity tangent entering
+     * through the same wet-face rule; divergence without the drift source
+     * (a function of occupancy only); the pressure tangent; gradient. */
+    forces_enforce(P, w->ct, w->vel);
+    divergence(P, w->vel, w->ct, w->occ, w->div);
+    orc_params Pe = *P;
+    Pe.tol = 1e-12f;
+    Pe.tol_sq = 1e-24f;
+    Pe.pressure_iters = 50000;
+    Pe.reduce_mode = 1;
+    orc_stats st;
+    memset(&st, 0, sizeof st);
+    FN(pressure_solve)(replay ? P : &Pe, w->ct, w->div, pr, &st);
+    gradient_enforce(P, pr, w->ct, w->vel);
+    real *ddiv = (real *)malloc(sizeof(real) * C);
+    real *dp = (real *)malloc(sizeof(real) * C);
+    for (uint32_t k = 0; k < K; ++k) {
+        real *dvk[3] = {dv[0] + (size_t)k * w->fc_alloc[0], dv[1] + (size_t)k * w->fc_alloc[1],
+                        dv[2] + (size_t)k * w->fc_alloc[2]};
+        orc_params Pk = *P;
+        for (int c = 0; c < 3; ++c) Pk.grav[c] = dgrav[3 * k + c];
+        forces_enforce(&Pk, w->ct, dvk);
+        orc_params P0 = *P;
+        P0.stiff = 0.0f;
+        divergence(&P0, dvk, w->ct, w->occ, ddiv);
+        memset(dp, 0, sizeof(real) * C);
+        /* The solve's guards are absolute (|d.q|, r.r > 1e-30), so the source
+         * is scaled by a power of two (exact) to order one first, and the
+         * solution scaled back; a zero source has the zero tangent. */
+        real mx = (real)0.0;
+        for (uint32_t c = 0; c < C; ++c) mx = RMAX(mx, RFABS(ddiv[c]));
+        if (mx > (real)0.0) {
+            const int e = (int)floor(log2((double)mx));
+            const real sc = (real)ldexp(1.0, -e), inv = (real)ldexp(1.0, e);
+            for (uint32_t c = 0; c < C; ++c) ddiv[c] *= sc;
+            orc_params Pt = Pe;
+            Pt.warm_start = 0u;
+            memset(&st, 0, sizeof st);
+            FN(pressure_solve)(&Pt, w->ct, ddiv, dp, &st);
+            for (uint32_t c = 0; c < C; ++c) dp[c] *= inv;
+        }
+        gradient_enforce(P, dp, w->ct, dvk);
+    }
+    free(ddiv);
+    free(dp);
+    /* G2P with the decisions of this state, and its tangent. */
+    const real h = (real)P->h, dt = (real)P->dt, alpha = (real)P->flip_blend;
+    const real maxs = RMAX((real)P->cfl, (real)1.0) * h / dt, hband = h * (real)1.05f;
+    real lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = (real)P->origin[c] + hband;
+        hi[c] = (real)P->origin[c] + (real)(float)(c == 0 ? P->nx : (c == 1 ? P->ny : P->nz)) * h - hband;
+    }
+#pragma omp parallel
+    {
+        /* per direction: dfin, dsav, dW of one axis; dnv3 (3 per direction);
+         * dpic3: dpic and dsv per axis (6 per direction) */
+        real *dfin = (real *)malloc(sizeof(real) * 3 * (K ? K : 1));
+        real *dsav = dfin + K, *dW = dfin + 2 * K;
+        real *dnv3 = (real *)malloc(sizeof(real) * 3 * (K ? K : 1));
+        real *dpic3 = (real *)malloc(sizeof(real) * 6 * (K ? K : 1));
+#pragma omp for schedule(static)
+        for (int64_t pp = 0; pp < (int64_t)n; ++pp) {
+            const real Pp[3] = {pos[0][pp], pos[1][pp], pos[2][pp]};
+            real pic[3], sv[3];
+            for (int a = 0; a < 3; ++a) {
+                uint32_t dim[3];
+                face_dims(P, a, dim);
+                int64_t base[3];
+                real t3[3];
+                for (int c = 0; c < 3; ++c) {
+                    const real g = (Pp[c] - (real)P->origin[c]) * inv_h + k_off[a][c];
+                    base[c] = sat_i32(RFLOOR(g));
+                    t3[c] = g - (real)base[c];
+                }
+                real fin = 0, sav = 0, W = 0;
+                for (uint32_t k = 0; k < K; ++k) dfin[k] = dsav[k] = dW[k] = (real)0.0;
+                for (int dk = 0; dk < 2; ++dk)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int di = 0; di < 2; ++di) {
+                            const int64_t ii = base[0] + di, jj = base[1] + dj, kk = base[2] + dk;
+                            if (ii < 0 || ii >= (int64_t)dim[0] || jj < 0 || jj >= (int64_t)dim[1] || kk < 0 ||
+                                kk >= (int64_t)dim[2])
+                                continue;
+                            if (touches_static_solid(P, a, ii, jj, kk)) continue;
+                            const int sel[3] = {di, dj, dk};
+                            real f[3];
+                            for (int c = 0; c < 3; ++c) f[c] = sel[c] ? t3[c] : (real)1.0 - t3[c];
+                            const real wgt = f[0] * f[1] * f[2];
+                            const uint64_t idx = (uint64_t)ii + dim[0] * ((uint64_t)jj + dim[1] * (uint64_t)kk);
+                            const real vf = w->vel[a][idx], sf = w->saved[a][idx];
+                            fin += wgt * vf;
+                            sav += wgt * sf;
+                            W += wgt;
+                            for (uint32_t k = 0; k < K; ++k) {
+                                real dw = (real)0.0;
+                                for (int c = 0; c < 3; ++c) {
+                                    const real dtc = dpos[3 * k + c][pp] * inv_h;
+                                    dw += (sel[c] ? dtc : -dtc) * f[(c + 1) % 3] * f[(c + 2) % 3];
+                                }
+                                const size_t ki = (size_t)k * w->fc_alloc[a] + idx;
+                                dfin[k] += dw * vf + wgt * dv[a][ki];
+                                dsav[k] += dw * sf + wgt * dsv[a][ki];
+                                dW[k] += dw;
+                            }
+                        }
+                pic[a] = W > (real)0.0 ? fin / W : (real)0.0;
+                sv[a] = W > (real)0.0 ? sav / W : (real)0.0;
+                for (uint32_t k = 0; k < K; ++k) {
+                    dpic3[6 * k + a] = W > (real)0.0 ? (dfin[k] - pic[a] * dW[k]) / W : (real)0.0;
+                    dpic3[6 * k + 3 + a] = W > (real)0.0 ? (dsav[k] - sv[a] * dW[k]) / W : (real)0.0;
+                }
+            }
+            real nv[3];
+            for (int a = 0; a < 3; ++a)
+                nv[a] = pic[a] * ((real)1.0 - alpha) + (vel[a][pp] + (pic[a] - sv[a])) * alpha;
+            for (uint32_t k = 0; k < K; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    const real dpk = dpic3[6 * k + a], dsk = dpic3[6 * k + 3 + a];
+                    dnv3[3 * k + a] = dpk * ((real)1.0 - alpha) + (dvel[3 * k + a][pp] + (dpk - dsk)) * alpha;
+                }
+            const real spd = RSQRT(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+            if (spd > maxs && spd > (real)0.0) {
+                /* nv -> nv maxs / |nv|: tangent (maxs/|nv|) (dnv - nv (nv . dnv) / |nv|^2) */
+                const real s = maxs / spd;
+                for (uint32_t k = 0; k < K; ++k) {
+                    const real nd = nv[0] * dnv3[3 * k] + nv[1] * dnv3[3 * k + 1] + nv[2] * dnv3[3 * k + 2];
+                    for (int a = 0; a < 3; ++a) dnv3[3 * k + a] = s * (dnv3[3 * k + a] - nv[a] * nd / (spd * spd));
+                }
+                for (int a = 0; a < 3; ++a) nv[a] = nv[a] * s;
+            }
+            for (int a = 0; a < 3; ++a) {
+                real np = Pp[a] + nv[a] * dt;
+                int clamped = 0;
+                if (np < lo[a]) {
+                    np = lo[a];
+                    nv[a] = (real)0.0;
+                    clamped = 1;
+                } else if (np > hi[a]) {
+                    np = hi[a];
+                    nv[a] = (real)0.0;
+                    clamped = 1;
+                }
+                pos[a][pp] = np;
+                vel[a][pp] = nv[a];
+                for (uint32_t k = 0; k < K; ++k) {
+                    if (clamped) {
+                        dpos[3 * k + a][pp] = (real)0.0;
+                        dvel[3 * k + a][pp] = (real)0.0;
+                    } else {
+                        dpos[3 * k + a][pp] = dpos[3 * k + a][pp] + dnv3[3 * k + a] * dt;
+                        dvel[3 * k + a][pp] = dnv3[3 * k + a];
+                    }
+                }
+            }
+        }
+        free(dfin);
+        free(dnv3);
+        free(dpic3);
+    }
+    for (int a = 0; a < 3; ++a) {
+        free(dv[a]);
+        free(dsv[a]);
+    }
+}
+
+/* Forward-mode directional derivatives along a trajectory computed
+ * elsewhere: substep s starts from states[s] (the layout of
+ * FN(adj_tape_from_states)); the K tangents (dpos, dvel: K x 3 arrays of n,
+ * in the input order of states[0]; dgrav: 3 K) are carried from substep to
+ * substep and leave in the order of the last substep's sort; replay selects
+ * the linearization point of tlm_substep. */
+void FN(tlm_from_states)(const orc_params *P, uint32_t n, uint32_t steps, const real *states, uint32_t K,
+                         real *const *dpos, real *const *dvel, const float *dgrav, int replay) {
+    adj_tape *T = adj_tape_new(P, n, 1, 1);
+    adj_ws w;
+    adj_ws_alloc(T, &w);
+    real *pos[3], *vel[3], *pr = (real *)malloc(sizeof(real) * T->C);
+    for (int a = 0; a < 3; ++a) {
+        pos[a] = (real *)malloc(sizeof(real) * (n ? n : 1));
+        vel[a] = (real *)malloc(sizeof(real) * (n ? n : 1));
+    }
+    for (uint32_t s = 0; s < steps; ++s) {
+        adj_load(T, states + (size_t)s * adj_snap_len(T), pos, vel, pr);
+        tlm_substep(P, n, pos, vel, pr, &w, K, dpos, dvel, dgrav, replay);
     }
     for (int a = 0; a < 3; ++a) {
         free(pos[a]);

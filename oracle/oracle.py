@@ -492,6 +492,34 @@ def smooth_record_states(P, states, f64=True):
     return rec
 
 
+def tlm_from_states(P, states, dpos0, dvel0, dgrav, replay=False, f64=True):
+    """orc_tlm_from_states_*: forward-mode (tangent-linear) derivatives of the
+    smooth forward along a trajectory computed elsewhere (states as for
+    smooth_record_states).  dpos0, dvel0: [K, N, 3] tangent directions of the
+    first input state (its order); dgrav: [K, 3] tangents of the gravity
+    vector.  replay=False linearizes each substep at the smooth substep (real
+    P2G sums, exact pressure solve); replay=True at the reverse pass's replay
+    of it (P2G at 2^22 fixed point, the substep's own CG tolerance), so that
+    forward mode and the reverse pass are products of one Jacobian.  Returns
+    (dpos, dvel), [K, N, 3], the tangents of the final state in the last
+    substep's sort order."""
+    real = np.float64 if f64 else np.float32
+    n = len(states[0][0])
+    K = len(dgrav)
+    cols_p = [np.array(dpos0[k][:, a], dtype=real, copy=True) for k in range(K) for a in range(3)]
+    cols_v = [np.array(dvel0[k][:, a], dtype=real, copy=True) for k in range(K) for a in range(3)]
+    g = np.ascontiguousarray(np.asarray(dgrav, np.float32).reshape(K * 3))
+    flat = np.concatenate([np.concatenate([np.asarray(p, real).T.ravel(), np.asarray(v, real).T.ravel(),
+                                           np.asarray(pr, real).ravel()]) for p, v, pr in states])
+    vp = C.c_void_p * (3 * K)
+    getattr(lib(), "orc_tlm_from_states_f64" if f64 else "orc_tlm_from_states_f32")(
+        C.byref(P), C.c_uint32(n), C.c_uint32(len(states)), ptr(flat), C.c_uint32(K), vp(*[ptr(c) for c in cols_p]),
+        vp(*[ptr(c) for c in cols_v]), ptr(g), C.c_int(1 if replay else 0))
+    dpos = np.stack([np.stack(cols_p[3 * k:3 * k + 3], 1) for k in range(K)])
+    dvel = np.stack([np.stack(cols_v[3 * k:3 * k + 3], 1) for k in range(K)])
+    return dpos, dvel
+
+
 # ---------------------------------------------------------------------------
 # Scene presets, spawn counts and the initial lattice (flip_oracle_scene.c).
 

@@ -169,7 +169,10 @@ def test_adjoint_at_64_cubed_against_fp64_and_frozen_differences():
     gravity_y) from the GPU reverse pass
     (a) against the FP64 reverse pass of the restatement along the GPU's own
         trajectory (measured 4e-7 on the gradient, 4e-6 per particle), and
-    (b) against central differences of the smooth forward that replays every
+    (b) against forward-mode derivatives at every input state of that
+        trajectory, linearized at the reverse pass's replay
+        (oracle.tlm_from_states), and
+    (c) against central differences of the smooth forward that replays every
         decision of that trajectory (sort order, cell types, occupancies,
         face supports, CFL and wall-clamp branches; stencils from the current
         positions) with real P2G sums and an exact pressure solve, at
@@ -203,6 +206,9 @@ def test_adjoint_at_64_cubed_against_fp64_and_frozen_differences():
     say("gpu", g_gpu, "fp64", g64)
     assert np.linalg.norm(g_gpu - g64) <= 1e-5 * np.linalg.norm(g64)
     assert rel_l2(gv, ogv) <= 1e-4
+    g_tlm = _tlm_gradient(P64, states, pf, True)
+    say("forward mode", g_tlm)
+    assert np.linalg.norm(g_gpu - g_tlm) <= 1e-5 * np.linalg.norm(g_tlm)
     rec = oracle.smooth_record_states(P64, states)
     del states
     x0, v0 = sc["pos"].astype(np.float64), sc["vel"].astype(np.float64)
@@ -222,10 +228,64 @@ def test_adjoint_at_64_cubed_against_fp64_and_frozen_differences():
         e[a] = 1.0
         fd.append((L(v0 + h * e, g0) - L(v0 - h * e, g0)) / (2 * h))
         say("frozen fd axis", a, fd[-1], "adjoint", g_gpu[a])
-    h = 1e-3
-    fd.append((L(v0, g0 + h) - L(v0, g0 - h)) / (2 * h))
+    # gravity is float32: divide by the step the rounded values take
+    gp, gm = float(np.float32(g0 + 1e-3)), float(np.float32(g0 - 1e-3))
+    fd.append((L(v0, gp) - L(v0, gm)) / (gp - gm))
     fd = np.array(fd)
     say("frozen fd", fd, "gpu", g_gpu)
     scale = np.linalg.norm(fd[:3])
     assert np.all(np.abs(fd[:3] - g_gpu[:3]) <= 2e-4 * scale)
     assert abs(fd[3] - g_gpu[3]) <= 5e-3 * abs(fd[3])
+
+
+def _tlm_gradient(P64, states, pf, replay):
+    """d loss / d (uniform velocity shift x, y, z; gravity_y) by forward mode
+    (oracle.tlm_from_states) along the given trajectory, linearized at the
+    reverse pass's replay of each substep or at the smooth substep."""
+    n = len(pf)
+    dpos0 = np.zeros((4, n, 3))
+    dvel0 = np.zeros((4, n, 3))
+    for a in range(3):
+        dvel0[a, :, a] = 1.0
+    dgrav = np.zeros((4, 3))
+    dgrav[3, 1] = 1.0
+    tp, _ = oracle.tlm_from_states(P64, states, dpos0, dvel0, dgrav, replay=replay)
+    up = upstream(pf)
+    return np.array([(up * tp[k]).sum() for k in range(4)])
+
+
+def test_adjoint_at_64_cubed_over_thirty_substeps_against_forward_mode():
+    """The 64^3 differentiable-control tank, 30 substeps, checkpoints every
+    10: the GPU reverse pass against forward-mode derivatives (dual numbers,
+    no step size) at every input state of the GPU's own trajectory, carried
+    through all 30 substeps.  These exist at horizons where central
+    differences no longer converge.  Linearized where the reverse pass
+    linearizes (its replay: P2G at 2^22, the substep's CG tolerance) forward
+    mode gives the same Jacobian's other product (bar 1e-4); linearized at
+    the smooth substep (real sums, exact solve) the derivative moves with the
+    linearization point (bar 1e-3)."""
+    sc = scenes.control_64()
+    params = gpu_params(sc, fsm.ENGINE_FUSED)
+    H = 30
+    say("adjoint 64^3 vs forward mode", len(sc["pos"]), "steps", H)
+    t = fsm.FluidStep(params, sc["pos"], sc["vel"])
+    t.forward_checkpointed(H, 10)
+    pos, _ = t.particles()
+    _, gv, gg = t.adjoint(upstream(pos), None)
+    t.close()
+    g_gpu = np.array([gv[:, 0].sum(), gv[:, 1].sum(), gv[:, 2].sum(), gg[1]], np.float64)
+    tt = fsm.FluidStep(params, sc["pos"], sc["vel"])
+    states = []
+    for _ in range(H):
+        p_, v_ = tt.particles()
+        states.append((p_, v_, tt.pressure()))
+        tt.step(1)
+    pf, _ = tt.particles()
+    tt.close()
+    assert np.array_equal(pf, pos)
+    P64 = oracle_params(sc, 1)
+    g_rep = _tlm_gradient(P64, states, pf, True)
+    g_tlm = _tlm_gradient(P64, states, pf, False)
+    say("gpu", g_gpu, "forward mode at the replay", g_rep, "at the smooth substep", g_tlm)
+    assert np.linalg.norm(g_gpu - g_rep) <= 1e-4 * np.linalg.norm(g_rep)
+    assert np.linalg.norm(g_gpu - g_tlm) <= 1e-3 * np.linalg.norm(g_tlm)

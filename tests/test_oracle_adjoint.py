@@ -237,3 +237,118 @@ def test_smooth_forward_record_then_replay_reproduces_the_trajectory():
     r = oracle.run(P, x0, v0, steps=4, sort_period=1, f64=True)
     assert np.abs(pa - r.pos).max() <= 1e-3
     assert rec.perm.shape == (4, len(x0)) and rec.support.any()
+
+
+def _tlm_directions(n, rng):
+    """Tangent directions: uniform velocity shifts x, y, z, gravity_y, and a
+    random per-particle position and velocity perturbation."""
+    dpos = np.zeros((5, n, 3))
+    dvel = np.zeros((5, n, 3))
+    for a in range(3):
+        dvel[a, :, a] = 1.0
+    dpos[4] = rng.standard_normal((n, 3)) * 1e-2
+    dvel[4] = rng.standard_normal((n, 3))
+    dgrav = np.zeros((5, 3))
+    dgrav[3, 1] = 1.0
+    return dpos, dvel, dgrav
+
+
+def test_tangent_linear_model_is_the_derivative_of_the_frozen_decision_forward():
+    """Forward mode (orc_tlm_from_states: every operation of the smooth
+    substep differentiated as a dual number, decisions taken from each input
+    state) against central differences of the smooth forward replaying the
+    same decisions, three substeps at 16^3, every particle's final position
+    and velocity.  Measured ~1e-9 relative at h = 1e-5 (x and z velocity
+    shifts, a random per-particle direction) and 2e-8 for gravity at 1e-3.
+    Gravity is a float32 parameter, so its step is the one the rounded
+    values g0 +- h actually take (at h = 1e-5 the f32 spacing near 9.81,
+    9.5e-7, moves an unrounded quotient by ~5%).  A uniform y shift is
+    removed by the projection (the liquid rests on the floor), so both sides
+    are ~0 there.  A dropped product-rule term, the wrong sign of a
+    tent-weight derivative or a missing gravity source each leave an O(1)
+    error."""
+    sc = scenes.small_tank(n=16, seed=5, fill=0.5, ppc=8, vel_scale=0.5)
+    P = oracle_params(sc, 1)
+    x0, v0 = sc["pos"].astype(np.float64), sc["vel"].astype(np.float64)
+    rec, states, _ = _smooth_trajectory(P, x0, v0, 3)
+    dpos0, dvel0, dgrav = _tlm_directions(len(x0), np.random.default_rng(3))
+    tp, tv = oracle.tlm_from_states(P, states, dpos0, dvel0, dgrav)
+    g0 = float(P.grav[1])
+
+    def forward(k, s, h):
+        gy = float(np.float32(g0 + s * h * dgrav[k, 1]))
+        Pg = oracle.make_params(P.nx, P.ny, P.nz, P.h, dt=P.dt, flip_blend=P.flip_blend, grav=(0.0, gy, 0.0),
+                                rest=P.rest, stiff=P.stiff, drift_clamp=P.drift_clamp, cfl=P.cfl,
+                                surface_dilation=P.surface_dilation, pressure_iters=P.pressure_iters, tol=P.tol,
+                                reduce_mode=1)
+        p_, v_, _ = oracle.smooth_forward(Pg, x0 + s * h * dpos0[k], v0 + s * h * dvel0[k], rec)
+        return p_, v_, gy
+
+    for k in range(5):
+        h = 1e-3 if k == 3 else 1e-5
+        (pa, va, ga), (pb, vb, gb) = forward(k, 1.0, h), forward(k, -1.0, h)
+        step = (ga - gb) if k == 3 else 2 * h
+        fp, fv = (pa - pb) / step, (va - vb) / step
+        if k == 1:
+            assert np.linalg.norm(tp[k]) <= 1e-8 and np.linalg.norm(fp) <= 1e-8
+            continue
+        assert np.linalg.norm(tp[k] - fp) <= 1e-6 * np.linalg.norm(fp), (k, np.linalg.norm(tp[k] - fp),
+                                                                         np.linalg.norm(fp))
+        assert np.linalg.norm(tv[k] - fv) <= 1e-6 * np.linalg.norm(fv), (k, np.linalg.norm(tv[k] - fv),
+                                                                         np.linalg.norm(fv))
+
+
+def test_tangent_linear_model_agrees_with_the_reverse_pass():
+    """The FP64 reverse pass and forward mode along the same trajectory
+    (24^3, twenty substeps at the default physics): the reverse pass's
+    Jacobian of the final centroid w.r.t. uniform velocity shifts and gravity
+    against the tangents' centroid.  Linearized at the same point (replay:
+    P2G at the reverse pass's 2^22 fixed point, the substep's own CG
+    tolerance) the two are products of one Jacobian and agree to rounding
+    (measured 1e-14): the reverse pass is the exact transpose of the forward
+    linearization.  Linearized at the smooth substep instead (real P2G sums,
+    exact solve) they differ by how much the Jacobian moves with the
+    linearization point, measured 1.5e-5 here."""
+    sc = scenes.small_tank(n=24, seed=9, fill=0.5, ppc=8, vel_scale=0.3)
+    P = oracle_params(sc, 1)
+    x0, v0 = sc["pos"].astype(np.float64), sc["vel"].astype(np.float64)
+    _, states, _ = _smooth_trajectory(P, x0, v0, 20)
+    T = oracle.AdjointTape.from_states(P, states)
+    n = len(x0)
+    J_ad = np.zeros((3, 4))
+    for i in range(3):
+        up = np.zeros((n, 3))
+        up[:, i] = 1.0 / n
+        _, gv, gg = T.backward(up, None)
+        J_ad[i] = [gv[:, 0].sum(), gv[:, 1].sum(), gv[:, 2].sum(), gg[1]]
+    T.close()
+    dpos0, dvel0, dgrav = _tlm_directions(n, np.random.default_rng(4))
+    for replay, bar in ((True, 1e-11), (False, 5e-5)):
+        tp, _ = oracle.tlm_from_states(P, states, dpos0[:4], dvel0[:4], dgrav[:4], replay=replay)
+        J_tlm = np.stack([tp[k].mean(0) for k in range(4)], 1)
+        assert np.linalg.norm(J_ad - J_tlm) <= bar * np.linalg.norm(J_tlm), (replay, J_ad, J_tlm)
+
+
+def test_dot_product_of_forward_mode_and_reverse_pass_per_substep():
+    """<w, J u> from forward mode equals <J^T w, u> from the reverse pass for
+    random full-state directions u (positions and velocities of every
+    particle) and cotangents w, one substep at each of several states of a
+    sixteen-substep run at 16^3, both linearized at the reverse pass's replay.
+    A transposed stencil, a missing term of one side or a wrong sign breaks
+    the identity by O(1); measured 5e-14 to 1.5e-12 relative."""
+    sc = scenes.small_tank(n=16, seed=5, fill=0.5, ppc=8, vel_scale=0.5)
+    P = oracle_params(sc, 1)
+    x0, v0 = sc["pos"].astype(np.float64), sc["vel"].astype(np.float64)
+    _, states, _ = _smooth_trajectory(P, x0, v0, 16)
+    n = len(x0)
+    rng = np.random.default_rng(11)
+    for s in (0, 7, 15):
+        u_p, u_v = rng.standard_normal((n, 3)) * 1e-2, rng.standard_normal((n, 3))
+        w_p, w_v = rng.standard_normal((n, 3)), rng.standard_normal((n, 3)) * 1e-2
+        tp, tv = oracle.tlm_from_states(P, [states[s]], u_p[None], u_v[None], np.zeros((1, 3)), replay=True)
+        lhs = (w_p * tp[0]).sum() + (w_v * tv[0]).sum()
+        T = oracle.AdjointTape.from_states(P, [states[s]])
+        gx, gv, _ = T.backward(w_p, w_v)
+        T.close()
+        rhs = (gx * u_p).sum() + (gv * u_v).sum()
+        assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (s, lhs, rhs)
