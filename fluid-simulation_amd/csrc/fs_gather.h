@@ -255,4 +255,247 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
     }
 }
 
+// ---- particle-owned form: every particle's taps evaluated once ----
+// One workgroup per 8x8x8 brick takes the particles whose sort cell lies in
+// the brick (one contiguous segment per (y,z) row of the brick after the
+// sort) and adds all 24 taps of each into LDS with integer atomics.  For axis
+// a the LDS region holds the faces such a particle can reach: face indices
+// [x0, x0 + 8] along a and [x0 - 1, x0 + 8] across it (9 x 10 x 10 faces), so
+// no ownership test and no halo walk is needed, and no tap is evaluated by
+// two workgroups.  A tap outside the grid adds 0 to a region slot that is
+// never flushed; a particle whose P2G cell differs from its sort cell (the
+// misfit of the gather above) or that lies outside the tank may reach past
+// the region, and only its taps take the per-tap path, adding out-of-region
+// taps to num/den with global atomics.  Afterwards the region's non-zero
+// sums are added to num/den with global i32 atomics: a face's total is the
+// sum of its terms over every brick that reached it, in any order (wrapping
+// integer adds), so num/den equal the reference's per-tap atomics bit for
+// bit.  num/den start cleared; the face epilogue runs as its own pass.
+constexpr int kSB = 8;                      // brick edge (cells)
+constexpr int kSRA = kSB + 1, kSRO = kSB + 2;  // region extent along / across the axis
+constexpr int kSRegion = kSRA * kSRO * kSRO;   // faces per axis
+constexpr uint32_t kSPerThread = 4;         // consecutive particle slots a thread takes per round
+constexpr int kSThreads = 256;              // workgroup size; small grids (<= kGSmallGrid bricks) use
+constexpr int kSThreadsSmall = 512;         // more lanes per brick
+constexpr uint32_t kSDefer = 256;           // particles per brick whose P2G cell is not in it, listed
+
+// The taps of one particle and axis whose stencil leaves the brick's region
+// (a misfit or a particle outside the tank), as p2g_taps: in-region taps into
+// LDS, the others straight into num/den with global atomics.  Rare.
+__device__ __forceinline__ void scatter_taps_outside(const StepParams& P, int a, const float g[3], float vcomp,
+                                                  int32_t x0, int32_t y0, int32_t z0, int32_t* s_num,
+                                                  int32_t* s_den, int32_t* num, int32_t* den) {
+    uint32_t dim[3];
+    face_dims(P, a, dim);
+    const int32_t rdx = a == 0 ? kSRA : kSRO, rdy = a == 1 ? kSRA : kSRO, rdz = a == 2 ? kSRA : kSRO;
+    const int32_t rx = x0 - (a != 0), ry = y0 - (a != 1), rz = z0 - (a != 2);
+    p2g_taps(g, dim, vcomp, P.scale, [&](int32_t ii, int32_t jj, int32_t kk, int32_t tn, int32_t td) {
+        const int32_t mx = ii - rx, my = jj - ry, mz = kk - rz;
+        if (mx >= 0 && mx < rdx && my >= 0 && my < rdy && mz >= 0 && mz < rdz) {
+            const int32_t li = a * kSRegion + mx + rdx * (my + rdy * mz);
+            atomicAdd(&s_num[li], tn);
+            atomicAdd(&s_den[li], td);
+        } else {
+            const uint32_t f = (uint32_t)ii + dim[0] * ((uint32_t)jj + dim[1] * (uint32_t)kk);
+            atomicAdd(&num[f], tn);
+            atomicAdd(&den[f], td);
+        }
+    });
+}
+
+template <int BF16, int THREADS>
+__device__ __forceinline__ void p2g_scatter_brick(const StepParams& P, const ParticleSoA& sp,
+                                                  const uint32_t* __restrict__ occ, const uint32_t* __restrict__ off,
+                                                  int32_t* const num[3], int32_t* const den[3], int32_t x0, int32_t y0,
+                                                  int32_t z0, const uint32_t* __restrict__ perm) {
+    __shared__ int32_t s_num[3 * kSRegion];
+    __shared__ int32_t s_den[3 * kSRegion];
+    __shared__ uint32_t s_row_start[kSB * kSB];
+    __shared__ uint32_t s_row_pref[kSB * kSB + 1];
+    __shared__ uint32_t s_defer[kSDefer];
+    __shared__ uint32_t s_ndefer;
+    const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
+    const int32_t xe = min(x0 + kSB - 1, nx - 1);
+    if (threadIdx.x < 64) {
+        // Row r = (y0 + r % 8, z0 + r / 8): its particles are those of cells
+        // [x0, xe] of that row, one segment of the sorted order.
+        const int r = (int)threadIdx.x;
+        const int32_t cy = y0 + r % kSB, cz = z0 + r / kSB;
+        uint32_t len = 0, start = 0;
+        if (cy < ny && cz < nz) {
+            const uint32_t cf = (uint32_t)x0 + P.nx * ((uint32_t)cy + P.ny * (uint32_t)cz);
+            const uint32_t cl = (uint32_t)xe + P.nx * ((uint32_t)cy + P.ny * (uint32_t)cz);
+            start = off[cf];
+            len = off[cl] + occ[cl] - start;
+        }
+        s_row_start[r] = start;
+        uint32_t inc = len;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (r >= o) inc += t;
+        }
+        s_row_pref[r + 1] = inc;
+        if (r == 0) {
+            s_row_pref[0] = 0;
+            s_ndefer = 0;
+        }
+    }
+    for (int t = threadIdx.x; t < 3 * kSRegion; t += THREADS) {
+        s_num[t] = 0;
+        s_den[t] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = s_row_pref[kSB * kSB];
+    if (total == 0) return;  // no particle: nothing to add (num/den start cleared)
+
+    // The particle's P2G cell floor((pos - origin) * inv_h) lies in the brick
+    // for a particle whose sort cell does, except a misfit or a particle
+    // outside the tank; then every tap of every axis has a region slot.
+    auto in_brick = [&](float px, float py, float pz) -> bool {
+        const float q[3] = {(px - P.origin[0]) * P.inv_h, (py - P.origin[1]) * P.inv_h,
+                            (pz - P.origin[2]) * P.inv_h};
+        const int32_t o[3] = {x0, y0, z0};
+        bool in = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int32_t cc = sat_i32(floorf(q[c]));
+            in = in && cc >= o[c] && cc < o[c] + kSB;
+        }
+        return in;
+    };
+    // All 24 taps of a particle whose P2G cell lies in the brick, into LDS.
+    auto add_particle = [&](const float px, const float py, const float pz, const float pv[3]) {
+        const float pos[3] = {(px - P.origin[0]) * P.inv_h, (py - P.origin[1]) * P.inv_h,
+                              (pz - P.origin[2]) * P.inv_h};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t dim[3];
+            face_dims(P, a, dim);
+            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
+            const int32_t bx = sat_i32(floorf(g[0])), by = sat_i32(floorf(g[1])), bz = sat_i32(floorf(g[2]));
+            const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
+            // Region of axis a: origin (x0, y0, z0) less one cell across the
+            // axis; extents kSRA along it, kSRO across.
+            const int32_t rdx = a == 0 ? kSRA : kSRO, rdy = a == 1 ? kSRA : kSRO;
+            const int32_t lx = bx - (x0 - (a != 0)), ly = by - (y0 - (a != 1)), lz = bz - (z0 - (a != 2));
+            bool gx[2], gy[2], gz[2];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                gx[d] = bx + d >= 0 && bx + d < (int32_t)dim[0];
+                gy[d] = by + d >= 0 && by + d < (int32_t)dim[1];
+                gz[d] = bz + d >= 0 && bz + d < (int32_t)dim[2];
+            }
+            const float vcomp = pv[a];
+            const float wx[2] = {1.0f - tx, tx}, wy[2] = {1.0f - ty, ty}, wz[2] = {1.0f - tz, tz};
+            float wxy[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wxy[q] = wx[q & 1] * wy[q >> 1];
+            const int32_t base = a * kSRegion + lx + rdx * (ly + rdy * lz);
+            // Fixed point as in p2g_taps (sh/scatter.wgsl:78-118): w in (0, 1]
+            // at scale 2^16 never saturates; the velocity term cannot when
+            // |v| * scale < 2e9 (false for NaN), else the saturating form.
+            // A tap that is skipped (outside the grid, or w <= 0) adds 0 to
+            // its slot instead of branching; slots outside the grid are never
+            // flushed.
+            auto add_taps = [&](auto plain_v) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
+                    const float w = wxy[t & 3] * wz[dk];  // wx * wy * wz
+                    const bool take = gx[di] && gy[dj] && gz[dk] && w > 0.0f;
+                    const float tv = rintf(w * vcomp * P.scale);
+                    const int32_t tn = take ? (decltype(plain_v)::value ? (int32_t)tv : sat_i32(tv)) : 0;
+                    const int32_t td = take ? (int32_t)rintf(w * P.scale) : 0;
+                    const int32_t li = base + di + rdx * (dj + rdy * dk);
+                    atomicAdd(&s_num[li], tn);
+                    atomicAdd(&s_den[li], td);
+                }
+            };
+            if (fabsf(vcomp) * P.scale < 2.0e9f)
+                add_taps(std::true_type{});
+            else
+                add_taps(std::false_type{});
+        }
+    };
+    // A particle whose P2G cell is not in the brick, tap by tap as p2g_taps:
+    // taps in the region into LDS, the others into num/den with global atomics.
+    auto add_outside = [&](uint32_t p) {
+        const float pos[3] = {(sp.pos[0][p] - P.origin[0]) * P.inv_h, (sp.pos[1][p] - P.origin[1]) * P.inv_h,
+                              (sp.pos[2][p] - P.origin[2]) * P.inv_h};
+        const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
+                             load_vel<BF16>(sp.vel[2], p)};
+        for (int a = 0; a < 3; ++a) {
+            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
+            scatter_taps_outside(P, a, g, pv[a], x0, y0, z0, s_num, s_den, num[a], den[a]);
+        }
+    };
+    // The brick's particle slots: each round gives a thread kSPerThread
+    // consecutive items, taken one after another; the lanes of a wave are
+    // kSPerThread * THREADS / 64 items apart, so they fall in different
+    // cells, whose taps mostly land on different LDS slots (same-address
+    // atomics within a wave serialize).
+    constexpr uint32_t kLaneStride = kSPerThread * (THREADS / 64);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    auto walk = [&](auto second) {
+        int row = 0;
+        for (uint32_t base = 0; base < total; base += THREADS * kSPerThread) {
+#pragma unroll 1
+            for (uint32_t m = 0; m < kSPerThread; ++m) {
+                const uint32_t t = base + lane * kLaneStride + wave * kSPerThread + m;
+                if (t >= total) break;
+                while (s_row_pref[row + 1] <= t) ++row;
+                uint32_t p = s_row_start[row] + (t - s_row_pref[row]);
+                if (perm) p = perm[p];
+                const float px = sp.pos[0][p], py = sp.pos[1][p], pz = sp.pos[2][p];
+                if (decltype(second)::value) {
+                    // Only when the deferred list overflowed: the particles
+                    // not in the list, tap by tap.
+                    if (in_brick(px, py, pz)) continue;
+                    bool listed = false;
+                    for (uint32_t e = 0; e < kSDefer; ++e) listed = listed || s_defer[e] == p;
+                    if (!listed) add_outside(p);
+                    continue;
+                }
+                if (!in_brick(px, py, pz)) {
+                    const uint32_t slot = atomicAdd(&s_ndefer, 1u);
+                    if (slot < kSDefer) s_defer[slot] = p;
+                    continue;
+                }
+                const float pv[3] = {load_vel<BF16>(sp.vel[0], p), load_vel<BF16>(sp.vel[1], p),
+                                     load_vel<BF16>(sp.vel[2], p)};
+                add_particle(px, py, pz, pv);
+            }
+        }
+    };
+    walk(std::false_type{});
+    __syncthreads();
+    // Particles whose P2G cell is not in the brick (rare): the list, and if it
+    // overflowed, a second walk for the rest.
+    const uint32_t nd = s_ndefer;
+    for (uint32_t e = threadIdx.x; e < (nd < kSDefer ? nd : kSDefer); e += THREADS) add_outside(s_defer[e]);
+    if (nd > kSDefer) walk(std::true_type{});
+    __syncthreads();
+    // Flush: the region's in-grid, non-zero sums into num/den (the axis loop
+    // is unrolled, so every array is picked at compile time).
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t rdx = a == 0 ? kSRA : kSRO, rdy = a == 1 ? kSRA : kSRO;
+        uint32_t dim[3];
+        face_dims(P, a, dim);
+        int32_t* __restrict__ na = num[a];
+        int32_t* __restrict__ da = den[a];
+        for (int li = threadIdx.x; li < kSRegion; li += THREADS) {
+            const int32_t n_ = s_num[a * kSRegion + li], d_ = s_den[a * kSRegion + li];
+            if ((n_ | d_) == 0) continue;
+            const int32_t i = x0 - (a != 0) + li % rdx, j = y0 - (a != 1) + (li / rdx) % rdy,
+                          k = z0 - (a != 2) + li / (rdx * rdy);
+            if (i < 0 || j < 0 || k < 0 || i >= (int32_t)dim[0] || j >= (int32_t)dim[1] || k >= (int32_t)dim[2])
+                continue;
+            const uint32_t f = (uint32_t)i + dim[0] * ((uint32_t)j + dim[1] * (uint32_t)k);
+            atomicAdd(&na[f], n_);
+            atomicAdd(&da[f], d_);
+        }
+    }
+}
+
 }  // namespace fs

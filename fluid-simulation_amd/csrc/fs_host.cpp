@@ -477,14 +477,24 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
             order = t->gperm;
             starts = t->goff;
         }
-        FS_HIP(launch_p2g_gather(P, from, from, t->bf16, t->occ, starts, t->ct, t->vel, t->saved,
-                                 t->counters + 1, t->mis_list, s, t->gather_brick, order));
-        // Fallback when more particles' P2G cells differ from their sort
-        // cells than the gather walks (any particle order will do).
-        FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
-        FS_HIP(launch_p2g_atomic(P, from, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
-        FS_MARK(t, kSecScatter);
-        FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap, s));
+        if (t->p2g_particle) {
+            // Particle-owned: each particle's taps added once, by the brick
+            // of its sort cell, into cleared num/den; then the face pass.
+            FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, nullptr, 0u, s));
+            FS_HIP(launch_p2g_bricks(P, from, t->bf16, t->occ, starts, t->num, t->den, order, s));
+            FS_MARK(t, kSecScatter);
+            FS_HIP(launch_face_epilogue_rows(P, t->num, t->den, t->ct, t->vel, t->saved, s));
+        } else {
+            FS_HIP(launch_p2g_gather(P, from, from, t->bf16, t->occ, starts, t->ct, t->vel, t->saved,
+                                     t->counters + 1, t->mis_list, s, t->gather_brick, order));
+            // Fallback when more particles' P2G cells differ from their sort
+            // cells than the gather walks (any particle order will do).
+            FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
+            FS_HIP(launch_p2g_atomic(P, from, t->bf16, t->num, t->den, t->counters + 1, kMisfitCap, s));
+            FS_MARK(t, kSecScatter);
+            FS_HIP(launch_face_epilogue(P, t->num, t->den, t->ct, t->vel, t->saved, t->counters + 1, kMisfitCap,
+                                        s));
+        }
         FS_MARK(t, kSecNormalize);
         t->gather_last = 1;
     } else {
@@ -800,6 +810,7 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         }
         if (const char* env = std::getenv("FLUIDSTEP_CG_MODE")) t->cg_mode = std::strcmp(env, "persistent") == 0 ? 0 : 1;
         if (const char* env = std::getenv("FLUIDSTEP_CG_GRAPH")) t->cg_use_graph = std::atoi(env) != 0;
+        if (const char* env = std::getenv("FLUIDSTEP_P2G_FORM")) t->p2g_particle = std::strcmp(env, "particle") == 0;
         if (const char* env = std::getenv("FLUIDSTEP_GATHER_BRICK")) {
             const int v = std::atoi(env);
             if (v >= 0 && v <= 2) t->gather_brick = v;
@@ -1119,7 +1130,7 @@ FS_API int fs_get_stats(fs_tank* t, fs_stats* out) {
     FS_HIP(hipMemcpy(cnt, t->counters, sizeof cnt, hipMemcpyDeviceToHost));
     s.liquid_cells = cnt[0];
     s.sorted = (uint32_t)t->sorted_last;
-    s.p2g_gather = (uint32_t)(t->gather_last && cnt[1] <= kMisfitCap);
+    s.p2g_gather = (uint32_t)(t->gather_last && (t->p2g_particle || cnt[1] <= kMisfitCap));
     if (t->prm.engine == FS_ENGINE_FUSED) {
         CgState st;
         FS_HIP(hipMemcpy(&st, t->cg.state, sizeof st, hipMemcpyDeviceToHost));

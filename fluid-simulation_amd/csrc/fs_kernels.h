@@ -101,6 +101,16 @@ hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const ui
 // gather does nothing when *mis_count exceeds kMisfitCap (fs_gather.h).  With
 // perm (the stable cell order of `sorted`, whose particles were not moved),
 // sorted slot d is particle perm[d].
+// The face epilogue (normalize, save, forces, enforce) of every face from
+// num/den, one row of faces per workgroup.
+hipError_t launch_face_epilogue_rows(const StepParams& P, const int32_t* const num[3], const int32_t* const den[3],
+                                     const uint8_t* ct, float* const vel[3], float* const saved[3], hipStream_t s);
+// Particle-owned P2G (fs_gather.h p2g_scatter_brick): every particle's taps
+// added once, by the brick of its sort cell; num/den cleared before, the face
+// epilogue after.  perm as for launch_p2g_gather.
+hipError_t launch_p2g_bricks(const StepParams& P, const ParticleSoA& sorted, int bf16, const uint32_t* occ,
+                             const uint32_t* cell_offset, int32_t* const num[3], int32_t* const den[3],
+                             const uint32_t* perm, hipStream_t s);
 hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
                              const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct,
                              float* const vel[3], float* const saved[3], const uint32_t* mis_count,

@@ -56,6 +56,7 @@ struct fs_tank {
     int cg_multi_blocks = 0;  // multi-kernel form
     int cg_use_graph = 1;     // replay the multi-kernel solve from a captured hipGraph
     int gather_brick = 0;     // P2G gather brick: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells
+    int p2g_particle = 0;     // 1: the particle-owned brick scatter instead of the face-owning gather
     hipGraphExec_t cg_exec = nullptr;
 
     uint64_t tick = 0;  // sort tick (advance_sort_tick, src/gpu/fluid.rs:1491-1498)

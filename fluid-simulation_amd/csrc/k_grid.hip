@@ -9,6 +9,7 @@
 //     kernel, and gradient + enforce as one pass over all faces.
 // Both forms evaluate the per-face functions of fs_face.h, so they agree bit
 // for bit.
+#include <cstdlib>
 #include "fs_cell.h"
 #include "fs_gather.h"
 
@@ -188,11 +189,12 @@ hipError_t launch_p2g_local(const StepParams& P, const ParticleSoA& src, int bf1
 }
 
 // Clears num and den of the three axes when *guard > above (the fused
-// engine's fallback to the atomic P2G): one launch for the six arrays.
+// engine's fallback to the atomic P2G), or always when guard is null: one
+// launch for the six arrays.
 __global__ void k_clear_faces_guarded(StepParams P, int32_t* __restrict__ n0, int32_t* __restrict__ n1,
                                       int32_t* __restrict__ n2, int32_t* __restrict__ d0, int32_t* __restrict__ d1,
                                       int32_t* __restrict__ d2, const uint32_t* __restrict__ guard, uint32_t above) {
-    if (*guard <= above) return;
+    if (guard && *guard <= above) return;
     const uint32_t tot = P.F[0] + P.F[1] + P.F[2];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < tot; gi += stride) {
@@ -207,7 +209,8 @@ __global__ void k_clear_faces_guarded(StepParams P, int32_t* __restrict__ n0, in
 hipError_t launch_clear_faces_guarded(const StepParams& P, int32_t* const num[3], int32_t* const den[3],
                                       const uint32_t* guard, uint32_t above, hipStream_t s) {
     const size_t tot = (size_t)P.F[0] + P.F[1] + P.F[2];
-    const unsigned need = grid_for(tot, 256), g = need < 1024u ? need : 1024u;
+    // Guarded (usually a no-op): a grid small enough to be resident at once.
+    const unsigned need = grid_for(tot, 256), g = guard && need > 1024u ? 1024u : need;
     FS_LAUNCH(k_clear_faces_guarded, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
               guard, above);
     return hipGetLastError();
@@ -295,6 +298,14 @@ hipError_t launch_divergence(const StepParams& P, float* const vel[3], const uin
 }
 
 // ---- fused face passes over the concatenated u|v|w face index space ----
+struct FaceIntsC {
+    const int32_t* num[3];
+    const int32_t* den[3];
+};
+struct FaceFloats {
+    float* p[3];
+};
+
 __global__ void k_face_epilogue(StepParams P, const int32_t* __restrict__ n0, const int32_t* __restrict__ n1,
                                 const int32_t* __restrict__ n2, const int32_t* __restrict__ d0,
                                 const int32_t* __restrict__ d1, const int32_t* __restrict__ d2,
@@ -332,6 +343,50 @@ hipError_t launch_face_epilogue(const StepParams& P, const int32_t* const num[3]
     if (guard && g > 1024) g = 1024;
     FS_LAUNCH(k_face_epilogue, dim3(g), dim3(256), 0, s, P, num[0], num[1], num[2], den[0], den[1], den[2],
                        ct, vel[0], vel[1], vel[2], saved[0], saved[1], saved[2], guard, above);
+    return hipGetLastError();
+}
+
+// The face epilogue of every face, one row of faces (fixed axis, j, k) per
+// workgroup: the face's (i, j, k) come from the launch shape (one division
+// per workgroup) instead of per-face divisions.
+constexpr int kRowThreads = 128;
+__global__ __launch_bounds__(kRowThreads) void k_face_epilogue_rows(StepParams P, FaceIntsC fi,
+                                                                     const uint8_t* __restrict__ ct, FaceFloats vel,
+                                                                     FaceFloats sav) {
+    // The axis and its face grid by selects, not by indexing the argument
+    // arrays with a run-time value (which would place them in scratch).
+    const uint32_t rows0 = P.ny * P.nz, rows1 = (P.ny + 1u) * P.nz;
+    uint32_t row = blockIdx.x;
+    const int a = row < rows0 ? 0 : (row < rows0 + rows1 ? 1 : 2);
+    row -= a == 0 ? 0u : (a == 1 ? rows0 : rows0 + rows1);
+    const uint32_t d0 = a == 0 ? P.nx + 1u : P.nx, d1 = a == 1 ? P.ny + 1u : P.ny;
+    const uint32_t j = row % d1, k = row / d1;
+    const int32_t* __restrict__ num = a == 0 ? fi.num[0] : (a == 1 ? fi.num[1] : fi.num[2]);
+    const int32_t* __restrict__ den = a == 0 ? fi.den[0] : (a == 1 ? fi.den[1] : fi.den[2]);
+    float* __restrict__ v = a == 0 ? vel.p[0] : (a == 1 ? vel.p[1] : vel.p[2]);
+    float* __restrict__ sv = a == 0 ? sav.p[0] : (a == 1 ? sav.p[1] : sav.p[2]);
+    const float g = a == 0 ? P.grav[0] : (a == 1 ? P.grav[1] : P.grav[2]);
+    for (uint32_t i = threadIdx.x; i < d0; i += kRowThreads) {
+        const uint32_t f = i + d0 * row;
+        const int32_t dd = den[f];
+        float vv = dd > 0 ? (float)num[f] / (float)dd : 0.0f;
+        sv[f] = vv;
+        if (face_wet(P, a, ct, i, j, k)) vv = vv + g * P.dt;
+        if (face_solid(P, a, ct, i, j, k)) vv = 0.0f;
+        v[f] = vv;
+    }
+}
+
+hipError_t launch_face_epilogue_rows(const StepParams& P, const int32_t* const num[3], const int32_t* const den[3],
+                                     const uint8_t* ct, float* const vel[3], float* const saved[3], hipStream_t s) {
+    uint32_t rows = 0;
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t d1 = a == 1 ? P.ny + 1u : P.ny, d2 = a == 2 ? P.nz + 1u : P.nz;
+        rows += d1 * d2;
+    }
+    const FaceIntsC fi = {{num[0], num[1], num[2]}, {den[0], den[1], den[2]}};
+    const FaceFloats fv = {{vel[0], vel[1], vel[2]}}, fs = {{saved[0], saved[1], saved[2]}};
+    FS_LAUNCH(k_face_epilogue_rows, dim3(rows), dim3(kRowThreads), 0, s, P, fi, ct, fv, fs);
     return hipGetLastError();
 }
 
@@ -405,6 +460,58 @@ hipError_t launch_gather_v(const StepParams& P, const ParticleSoA& sorted, const
     return hipGetLastError();
 }
 }  // namespace
+
+// Particle-owned brick scatter (fs_gather.h), one workgroup per 8^3 brick;
+// num/den must be cleared before it and the face epilogue run after it.
+struct FaceInts {
+    int32_t* num[3];
+    int32_t* den[3];
+};
+
+template <int BF16, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_p2g_bricks(StepParams P, ParticleSoA sp, const uint32_t* __restrict__ occ,
+                                                         const uint32_t* __restrict__ off, FaceInts fi, uint32_t nbx,
+                                                         uint32_t nby, uint32_t nbz,
+                                                         const uint32_t* __restrict__ perm) {
+    const uint32_t G = nbx * nby * nbz;
+    const uint32_t b = xcd_remap(blockIdx.x, G);
+    const uint32_t bxi = b % nbx, byi = (b / nbx) % nby, bzi = b / (nbx * nby);
+    p2g_scatter_brick<BF16, THREADS>(P, sp, occ, off, fi.num, fi.den, (int32_t)bxi * kSB, (int32_t)byi * kSB,
+                                     (int32_t)bzi * kSB, perm);
+}
+
+namespace {
+template <int THREADS>
+hipError_t launch_bricks_t(const StepParams& P, const ParticleSoA& sorted, int bf16, const uint32_t* occ,
+                           const uint32_t* cell_offset, const FaceInts& fi, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                           const uint32_t* perm, hipStream_t s) {
+    const uint32_t G = nbx * nby * nbz;
+    if (bf16)
+        FS_LAUNCH((k_p2g_bricks<1, THREADS>), dim3(G), dim3(THREADS), 0, s, P, sorted, occ, cell_offset, fi, nbx, nby,
+                  nbz, perm);
+    else
+        FS_LAUNCH((k_p2g_bricks<0, THREADS>), dim3(G), dim3(THREADS), 0, s, P, sorted, occ, cell_offset, fi, nbx, nby,
+                  nbz, perm);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_p2g_bricks(const StepParams& P, const ParticleSoA& sorted, int bf16, const uint32_t* occ,
+                             const uint32_t* cell_offset, int32_t* const num[3], int32_t* const den[3],
+                             const uint32_t* perm, hipStream_t s) {
+    const uint32_t nbx = (P.nx + kSB - 1) / kSB, nby = (P.ny + kSB - 1) / kSB, nbz = (P.nz + kSB - 1) / kSB;
+    const FaceInts fi = {{num[0], num[1], num[2]}, {den[0], den[1], den[2]}};
+    int threads = nbx * nby * nbz <= kGSmallGrid ? kSThreadsSmall : kSThreads;
+    // FLUIDSTEP_P2G_THREADS=256|512|1024 fixes the workgroup size (read at
+    // every launch).
+    if (const char* e = std::getenv("FLUIDSTEP_P2G_THREADS")) {
+        const int v = std::atoi(e);
+        if (v == 256 || v == 512 || v == 1024) threads = v;
+    }
+    if (threads == 1024) return launch_bricks_t<1024>(P, sorted, bf16, occ, cell_offset, fi, nbx, nby, nbz, perm, s);
+    if (threads == 512) return launch_bricks_t<512>(P, sorted, bf16, occ, cell_offset, fi, nbx, nby, nbz, perm, s);
+    return launch_bricks_t<256>(P, sorted, bf16, occ, cell_offset, fi, nbx, nby, nbz, perm, s);
+}
 
 hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, const ParticleSoA& src, int bf16,
                              const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct, float* const vel[3],

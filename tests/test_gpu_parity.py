@@ -163,17 +163,21 @@ def test_non_power_of_two_grid_with_misfit(engine):
     t.close()
 
 
-def test_misfit_list_overflow_takes_atomic_form():
-    """More misfit particles than the gather's list holds (4096): the fused
-    engine switches to the atomic P2G form for the step; results match."""
+@pytest.mark.parametrize("form", ["owner", "particle"])
+def test_many_misfits(monkeypatch, form):
+    """More misfit particles than the face-owning gather's list holds (4096).
+    The face-owning gather (the default) switches to the atomic P2G form for
+    the step; the particle-owned brick scatter (FLUIDSTEP_P2G_FORM=particle)
+    lists them per brick and adds their taps one by one.  Results match."""
+    monkeypatch.setenv("FLUIDSTEP_P2G_FORM", form)
     sc = scenes.small_tank(n=24, seed=12, fill=0.45)
     pos = sc["pos"].copy()
     assert len(pos) > 6000
     pos[:5000, 1] = _brick_edge_misfit_24()
-    say("misfit overflow", sc["name"], "N", len(pos))
+    say("misfit overflow", sc["name"], "N", len(pos), "form", form)
     ref, t = run_pair(sc, fsm.ENGINE_FUSED, pos=pos)
     st = t.stats()
-    assert st["misfits"] >= 5000 and st["p2g_gather"] == 0
+    assert st["misfits"] >= 5000 and st["p2g_gather"] == (1 if form == "particle" else 0)
     assert np.array_equal(t.permutation(), ref.perm)
     saved = t.faces(fsm.FACE_SAVED)
     for a in range(3):
@@ -463,3 +467,32 @@ def test_unsorted_substeps_gather_equals_atomic(monkeypatch, n, seed, sort, peri
         pg, vg = g.particles()
         assert bits_equal(pa, pg) and bits_equal(va, vg)
     a.close(), g.close()
+
+
+@pytest.mark.parametrize("scene_name,sort,period", [("dam_break_32", 1, 1), ("high_density_128", 1, 1),
+                                                    ("small_tank_24", 1, 4)])
+def test_p2g_forms_give_identical_bits(monkeypatch, scene_name, sort, period):
+    """The fused engine's two P2G forms - the particle-owned brick scatter
+    (each particle's taps added once, by the brick of its sort cell, then a
+    face pass) and the face-owning gather (each brick sums the taps of the
+    particles of its halo that reach its faces) - add the same integer terms,
+    so faces, pressure and particles agree bit for bit over four substeps,
+    on sorted and (period 4) unsorted substeps."""
+    if scene_name == "small_tank_24":
+        sc = scenes.small_tank(n=24, seed=21, fill=0.5)
+    else:
+        sc = getattr(scenes, scene_name)()
+    outs = []
+    for form in ("owner", "particle"):
+        monkeypatch.setenv("FLUIDSTEP_P2G_FORM", form)
+        say("p2g form", scene_name, form)
+        t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED, particle_sort=sort, particle_sort_period=period),
+                          sc["pos"], sc["vel"])
+        arrs = []
+        for _ in range(4):
+            t.step(1)
+            arrs += [*t.faces(fsm.FACE_SAVED), *t.faces(fsm.FACE_VELOCITY), t.pressure(), *t.particles()]
+        outs.append(arrs)
+        t.close()
+    for i, (a, b) in enumerate(zip(*outs)):
+        assert bits_equal(a, b), i
