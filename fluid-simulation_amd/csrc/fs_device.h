@@ -64,10 +64,24 @@ __device__ __forceinline__ uint32_t cell_key(const StepParams& P, float x, float
     return cell_ijk(P, x, y, z, i, j, k);
 }
 
+// Per-axis parameters by selects: indexing a parameter array (or a local
+// one) with a run-time axis makes the compiler copy it to per-lane scratch
+// memory, which costs a whole pass of scratch traffic in a per-face kernel.
 __device__ __forceinline__ void face_dims(const StepParams& P, int a, uint32_t d[3]) {
-    d[0] = P.nx; d[1] = P.ny; d[2] = P.nz;
-    d[a] += 1u;
+    d[0] = P.nx + (a == 0 ? 1u : 0u);
+    d[1] = P.ny + (a == 1 ? 1u : 0u);
+    d[2] = P.nz + (a == 2 ? 1u : 0u);
 }
+// One of three uniform values by a run-time index.  The empty asm makes the
+// three opaque, so the compiler cannot fold the selects back into an indexed
+// load of the array they came from.
+template <typename T>
+__device__ __forceinline__ T pick3(int a, T x0, T x1, T x2) {
+    asm volatile("" : "+s"(x0), "+s"(x1), "+s"(x2));
+    return a == 0 ? x0 : (a == 1 ? x1 : x2);
+}
+__device__ __forceinline__ float grav_axis(const StepParams& P, int a) { return pick3(a, P.grav[0], P.grav[1], P.grav[2]); }
+__device__ __forceinline__ uint32_t faces_axis(const StepParams& P, int a) { return pick3(a, P.F[0], P.F[1], P.F[2]); }
 
 // Per-axis staggering of the MAC gather/scatter (sh/scatter.wgsl:97-116).
 __device__ __forceinline__ float stagger(int a, int c) { return a == c ? 0.0f : -0.5f; }

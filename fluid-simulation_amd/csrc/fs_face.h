@@ -60,7 +60,7 @@ __device__ __forceinline__ void face_epilogue(const StepParams& P, int a, const 
                                               float& vel, float& saved) {
     float v = den > 0 ? (float)num / (float)den : 0.0f;
     saved = v;
-    if (face_wet(P, a, ct, i, j, k)) v = v + P.grav[a] * P.dt;
+    if (face_wet(P, a, ct, i, j, k)) v = v + grav_axis(P, a) * P.dt;
     if (face_solid(P, a, ct, i, j, k)) v = 0.0f;
     vel = v;
 }

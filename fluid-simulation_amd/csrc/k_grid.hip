@@ -232,16 +232,16 @@ __global__ void k_copy(const float* __restrict__ src, float* __restrict__ dst, s
 
 __global__ void k_forces(StepParams P, int a, const uint8_t* __restrict__ ct, float* __restrict__ vel) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= P.F[a]) return;
+    if (f >= faces_axis(P, a)) return;
     uint32_t d[3], i, j, k;
     face_dims(P, a, d);
     face_ijk(f, d, i, j, k);
-    if (face_wet(P, a, ct, i, j, k)) vel[f] = vel[f] + P.grav[a] * P.dt;
+    if (face_wet(P, a, ct, i, j, k)) vel[f] = vel[f] + grav_axis(P, a) * P.dt;
 }
 
 __global__ void k_enforce(StepParams P, int a, const uint8_t* __restrict__ ct, float* __restrict__ vel) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= P.F[a]) return;
+    if (f >= faces_axis(P, a)) return;
     uint32_t d[3], i, j, k;
     face_dims(P, a, d);
     face_ijk(f, d, i, j, k);
@@ -251,7 +251,7 @@ __global__ void k_enforce(StepParams P, int a, const uint8_t* __restrict__ ct, f
 __global__ void k_gradient(StepParams P, int a, const float* __restrict__ pr, const uint8_t* __restrict__ ct,
                            float* __restrict__ vel) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= P.F[a]) return;
+    if (f >= faces_axis(P, a)) return;
     uint32_t d[3], i, j, k;
     face_dims(P, a, d);
     face_ijk(f, d, i, j, k);
@@ -361,11 +361,11 @@ __global__ __launch_bounds__(kRowThreads) void k_face_epilogue_rows(StepParams P
     row -= a == 0 ? 0u : (a == 1 ? rows0 : rows0 + rows1);
     const uint32_t d0 = a == 0 ? P.nx + 1u : P.nx, d1 = a == 1 ? P.ny + 1u : P.ny;
     const uint32_t j = row % d1, k = row / d1;
-    const int32_t* __restrict__ num = a == 0 ? fi.num[0] : (a == 1 ? fi.num[1] : fi.num[2]);
-    const int32_t* __restrict__ den = a == 0 ? fi.den[0] : (a == 1 ? fi.den[1] : fi.den[2]);
-    float* __restrict__ v = a == 0 ? vel.p[0] : (a == 1 ? vel.p[1] : vel.p[2]);
-    float* __restrict__ sv = a == 0 ? sav.p[0] : (a == 1 ? sav.p[1] : sav.p[2]);
-    const float g = a == 0 ? P.grav[0] : (a == 1 ? P.grav[1] : P.grav[2]);
+    const int32_t* __restrict__ num = pick3(a, fi.num[0], fi.num[1], fi.num[2]);
+    const int32_t* __restrict__ den = pick3(a, fi.den[0], fi.den[1], fi.den[2]);
+    float* __restrict__ v = pick3(a, vel.p[0], vel.p[1], vel.p[2]);
+    float* __restrict__ sv = pick3(a, sav.p[0], sav.p[1], sav.p[2]);
+    const float g = grav_axis(P, a);
     for (uint32_t i = threadIdx.x; i < d0; i += kRowThreads) {
         const uint32_t f = i + d0 * row;
         const int32_t dd = den[f];
