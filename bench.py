@@ -96,6 +96,25 @@ def run_gpu(fsm, sc, params, steps, warmup, timing_steps=3):
     return dt, per
 
 
+def section_model_bytes(N, C, F, c_act, iters):
+    """Compulsory bytes of one substep per fine section, by the byte model of
+    the step (140 N + 28 F + 35 C outside the CG, FP32 SoA particles, u8 cell
+    flags): mark 16 N + 4 C (position read, key write; histogram); sort 12 C
+    (scan and bin starts); classify 5 C; scatter (P2G and its face epilogue)
+    24 N + 8 F; divergence (with CG init) 4 F + 9 C + 21 C; gradient 8 F + 5 C;
+    g2p 48 N + 8 F plus the sort's 52 N particle permutation, which G2P
+    applies as it writes; cg_solve 45 B per liquid-tile cell and iteration
+    plus 9 B per cell for the f64 true residual."""
+    return {"mark": 16 * N + 4 * C, "sort": 12 * C, "classify": 5 * C, "scatter": 24 * N + 8 * F,
+            "divergence": 4 * F + 30 * C, "gradient_u": 8 * F + 5 * C, "g2p": 100 * N + 8 * F,
+            "cg_solve": 45 * c_act * iters + 9 * c_act}
+
+
+def section_gbs(fine, model):
+    """Achieved GB/s of each fine section on its compulsory bytes."""
+    return {k: round(model[k] / (fine[k] * 1e-3) / 1e9, 1) for k in model if fine.get(k, 0) > 0}
+
+
 def fine_sections(d):
     """One substep's device time by the reference's FINE_SECTIONS names (the
     fused engine's absorbed sections omitted) and its CG as one solve."""
@@ -746,6 +765,9 @@ def main():
                                           if s["rs_initial"] > 0 else 0.0 for s in per]},
         "sections_ms": {k: per[-1][k] for k in ("ms_prep", "ms_pressure", "ms_finish", "ms_cg")},
         "fine_sections_ms": fine_sections(per[-1]["detailed"]),
+        "fine_sections_gbs": section_gbs(fine_sections(per[-1]["detailed"]),
+                                         section_model_bytes(N, C, 3 * (sc["n"] + 1) * sc["n"] ** 2,
+                                                             per[-1]["cg_active_cells"], per[-1]["cg_iterations"])),
         "dispatches_per_substep": per[-1]["dispatches_per_substep"],
         "roofline": {
             "kernel": "k_cg_persistent",
