@@ -168,6 +168,7 @@ struct CgWork {
     uint32_t* n_tiles;       // [1]
     CgState* state;          // [1]
     long long spin_ticks;    // limit of every bounded wait (wall_clock64 ticks)
+    uint32_t tiles_per_block;  // persistent solve: most tiles a workgroup takes (0: one batch per wave)
 };
 // tiles[scan[t]] = t for flagged tiles; *n_active = the flagged count.
 hipError_t launch_compact_tiles(uint32_t ntiles, const uint32_t* flags, const uint32_t* scan, uint32_t* tiles,
