@@ -37,12 +37,14 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) {
     return fminf(fmaxf(x, lo), hi);  // WGSL clamp; fmin/fmax drop a NaN operand
 }
 
-// WGSL i32(f32): saturating, NaN -> 0 (v_cvt_i32_f32 behaves the same).
+// WGSL i32(f32): truncating, saturating, NaN -> 0.  v_cvt_i32_f32 does
+// exactly that (out-of-range values clamp to INT_MIN/INT_MAX, NaN gives 0),
+// so the conversion is that one instruction; the C cast is undefined out of
+// range and written with its checks compiled to three nested branches.
 __device__ __forceinline__ int32_t sat_i32(float x) {
-    if (!(x == x)) return 0;
-    if (x >= 2147483647.0f) return 2147483647;
-    if (x <= -2147483648.0f) return (-2147483647 - 1);
-    return (int32_t)x;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // Cell key of sh/mark.wgsl:35-43: g = (pos - origin) / h (division), clamped

@@ -18,51 +18,72 @@ __device__ __forceinline__ uint32_t axis_n(const StepParams& P, int a) {
 }
 
 // Face is wet if its lo or hi in-range cell is Liquid (sh/forces.wgsl:56-73).
-__device__ __forceinline__ bool face_wet(const StepParams& P, int a, const uint8_t* __restrict__ ct,
-                                         uint32_t i, uint32_t j, uint32_t k) {
+// cell(ci, cj, ck) returns the type of an in-grid cell.
+template <typename CellFn>
+__device__ __forceinline__ bool face_wet_by(const StepParams& P, int a, uint32_t i, uint32_t j, uint32_t k,
+                                            CellFn&& cell) {
     const uint32_t comp = a == 0 ? i : (a == 1 ? j : k);
     bool liquid = false;
     if (comp > 0u) {
         uint32_t ci = i, cj = j, ck = k;
         if (a == 0) ci -= 1u; else if (a == 1) cj -= 1u; else ck -= 1u;
-        if (ct[ci + P.nx * (cj + P.ny * ck)] == kLiquid) liquid = true;
+        if (cell(ci, cj, ck) == kLiquid) liquid = true;
     }
     if (comp < axis_n(P, a)) {
-        if (ct[i + P.nx * (j + P.ny * k)] == kLiquid) liquid = true;
+        if (cell(i, j, k) == kLiquid) liquid = true;
     }
     return liquid;
 }
 
 // Face touches a Solid cell or the domain edge (sh/boundaries.wgsl:39-58).
-__device__ __forceinline__ bool face_solid(const StepParams& P, int a, const uint8_t* __restrict__ ct,
-                                           uint32_t i, uint32_t j, uint32_t k) {
+template <typename CellFn>
+__device__ __forceinline__ bool face_solid_by(const StepParams& P, int a, uint32_t i, uint32_t j, uint32_t k,
+                                              CellFn&& cell) {
     const uint32_t comp = a == 0 ? i : (a == 1 ? j : k);
     bool lo_solid = comp == 0u;
     bool hi_solid = comp == axis_n(P, a);
     if (!lo_solid) {
         uint32_t ci = i, cj = j, ck = k;
         if (a == 0) ci -= 1u; else if (a == 1) cj -= 1u; else ck -= 1u;
-        if (ct[ci + P.nx * (cj + P.ny * ck)] == kSolid) lo_solid = true;
+        if (cell(ci, cj, ck) == kSolid) lo_solid = true;
     }
     if (!hi_solid) {
         uint32_t ci = i < P.nx - 1u ? i : P.nx - 1u;
         uint32_t cj = j < P.ny - 1u ? j : P.ny - 1u;
         uint32_t ck = k < P.nz - 1u ? k : P.nz - 1u;
-        if (ct[ci + P.nx * (cj + P.ny * ck)] == kSolid) hi_solid = true;
+        if (cell(ci, cj, ck) == kSolid) hi_solid = true;
     }
     return lo_solid || hi_solid;
 }
 
 // normalize -> save -> forces -> enforce for one face (sh/normalize.wgsl:21-26,
 // sh/save_vel.wgsl:18, sh/forces.wgsl:75-83, sh/boundaries.wgsl:58-60).
+template <typename CellFn>
+__device__ __forceinline__ void face_epilogue_by(const StepParams& P, int a, uint32_t i, uint32_t j, uint32_t k,
+                                                 int32_t num, int32_t den, float& vel, float& saved, CellFn&& cell) {
+    float v = den > 0 ? (float)num / (float)den : 0.0f;
+    saved = v;
+    if (face_wet_by(P, a, i, j, k, cell)) v = v + grav_axis(P, a) * P.dt;
+    if (face_solid_by(P, a, i, j, k, cell)) v = 0.0f;
+    vel = v;
+}
+
+// The same with the cell types read from the grid's array.
+__device__ __forceinline__ bool face_wet(const StepParams& P, int a, const uint8_t* __restrict__ ct,
+                                         uint32_t i, uint32_t j, uint32_t k) {
+    return face_wet_by(P, a, i, j, k,
+                       [&](uint32_t ci, uint32_t cj, uint32_t ck) { return ct[ci + P.nx * (cj + P.ny * ck)]; });
+}
+__device__ __forceinline__ bool face_solid(const StepParams& P, int a, const uint8_t* __restrict__ ct,
+                                           uint32_t i, uint32_t j, uint32_t k) {
+    return face_solid_by(P, a, i, j, k,
+                         [&](uint32_t ci, uint32_t cj, uint32_t ck) { return ct[ci + P.nx * (cj + P.ny * ck)]; });
+}
 __device__ __forceinline__ void face_epilogue(const StepParams& P, int a, const uint8_t* __restrict__ ct,
                                               uint32_t i, uint32_t j, uint32_t k, int32_t num, int32_t den,
                                               float& vel, float& saved) {
-    float v = den > 0 ? (float)num / (float)den : 0.0f;
-    saved = v;
-    if (face_wet(P, a, ct, i, j, k)) v = v + grav_axis(P, a) * P.dt;
-    if (face_solid(P, a, ct, i, j, k)) v = 0.0f;
-    vel = v;
+    face_epilogue_by(P, a, i, j, k, num, den, vel, saved,
+                     [&](uint32_t ci, uint32_t cj, uint32_t ck) { return ct[ci + P.nx * (cj + P.ny * ck)]; });
 }
 
 // Gradient subtraction for one face (sh/gradient.wgsl:41-66); returns the

@@ -50,14 +50,16 @@ struct GatherOut {
 // Brick at cell (x0, y0, z0).  sp: the sorted side; src: the side the keys
 // were computed on (listed indices refer to it); [pbeg, pend): the particle
 // range of this tank in both sides.  With perm, sp is not sorted: sorted slot
-// d holds particle perm[d] of sp (the particles were not moved).
+// d holds particle perm[d] of sp (the particles were not moved).  weight_out
+// (nullable) receives the number of particles the brick's halo holds.
 template <int BF16, int V, int THREADS = kGThreads>
 __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const ParticleSoA& sp, const ParticleSoA& src,
                                                  const uint32_t* __restrict__ occ, const uint32_t* __restrict__ off,
                                                  const uint8_t* __restrict__ ct, const GatherOut& out, int32_t x0,
                                                  int32_t y0, int32_t z0, const uint32_t* __restrict__ mis_list,
                                                  uint32_t mis_n, uint32_t pbeg, uint32_t pend,
-                                                 const uint32_t* __restrict__ perm = nullptr) {
+                                                 const uint32_t* __restrict__ perm = nullptr,
+                                                 uint32_t* __restrict__ weight_out = nullptr) {
     constexpr int kGBX = Brick<V>::X, kGBY = Brick<V>::Y, kGBZ = Brick<V>::Z;
     constexpr int kGRowsMax = (kGBY + 2) * (kGBZ + 2);
     constexpr int kGU = (kGBX + 1) * kGBY * kGBZ;
@@ -67,11 +69,24 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
     __shared__ int32_t s_den[kGU + kGV + kGW];
     __shared__ uint32_t s_row_start[kGRowsMax];
     __shared__ uint32_t s_row_pref[kGRowsMax + 1];
+    // Cell types of the brick grown by one cell on every side: every cell the
+    // epilogue's wet and solid tests read for an owned face (cells outside
+    // the grid are never read).  Loaded here, where their latency overlaps the
+    // row-segment loads, so the epilogue waits on LDS only.
+    constexpr int kCX = kGBX + 2, kCY = kGBY + 2, kCZ = kGBZ + 2;
+    __shared__ uint8_t s_ct[kCX * kCY * kCZ];
     const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
 
     for (int t = threadIdx.x; t < kGU + kGV + kGW; t += THREADS) {
         s_num[t] = 0;
         s_den[t] = 0;
+    }
+    for (int c = threadIdx.x; c < kCX * kCY * kCZ; c += THREADS) {
+        const int32_t ci = x0 - 1 + c % kCX, cj = y0 - 1 + (c / kCX) % kCY, ck = z0 - 1 + c / (kCX * kCY);
+        uint8_t v = 0xFFu;
+        if (ci >= 0 && ci < nx && cj >= 0 && cj < ny && ck >= 0 && ck < nz)
+            v = ct[(uint32_t)ci + P.nx * ((uint32_t)cj + P.ny * (uint32_t)ck)];
+        s_ct[c] = v;
     }
     // Halo cell range (inclusive), clamped to the grid.
     const int32_t xs = max(x0 - 1, 0), xe = min(x0 + kGBX, nx - 1);
@@ -105,6 +120,7 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
     }
     __syncthreads();
     const uint32_t total = s_row_pref[rows];
+    if (weight_out && threadIdx.x == 0) *weight_out = total;
     const int32_t ownx_hi = (x0 + kGBX >= nx) ? kGBX : kGBX - 1;  // own-axis local max index per axis
     const int32_t owny_hi = (y0 + kGBY >= ny) ? kGBY : kGBY - 1;
     const int32_t ownz_hi = (z0 + kGBZ >= nz) ? kGBZ : kGBZ - 1;
@@ -112,58 +128,80 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
     auto add_particle = [&](const float px, const float py, const float pz, const float pv[3]) {
         const float pos[3] = {(px - P.origin[0]) * P.inv_h, (py - P.origin[1]) * P.inv_h,
                               (pz - P.origin[2]) * P.inv_h};
+        // Below 2^24 grid units every g - floor(g) is exact and in [0, 1], so
+        // every weight is in [0, 1].
+        const bool moderate = fabsf(pos[0]) < 16777216.0f && fabsf(pos[1]) < 16777216.0f &&
+                              fabsf(pos[2]) < 16777216.0f;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             uint32_t dim[3];
             face_dims(P, a, dim);
-            const float g[3] = {pos[0] + stagger(a, 0), pos[1] + stagger(a, 1), pos[2] + stagger(a, 2)};
-            const int32_t hx = a == 0 ? ownx_hi : kGBX - 1;
-            const int32_t hy = a == 1 ? owny_hi : kGBY - 1;
-            const int32_t hz = a == 2 ? ownz_hi : kGBZ - 1;
+            // pos + stagger; the reference's + 0.0 on the own axis is left
+            // out: it only turns -0 into +0, which gives the same integers.
+            const float g[3] = {a == 0 ? pos[0] : pos[0] - 0.5f, a == 1 ? pos[1] : pos[1] - 0.5f,
+                                a == 2 ? pos[2] : pos[2] - 0.5f};
             const int32_t lx_dim = a == 0 ? kGBX + 1 : kGBX;
             const int32_t ly_dim = a == 1 ? kGBY + 1 : kGBY;
-            const int32_t base = a == 0 ? 0 : (a == 1 ? kGU : kGU + kGV);
+            const uint32_t base = a == 0 ? 0u : (a == 1 ? (uint32_t)kGU : (uint32_t)(kGU + kGV));
+            // A face index c is owned when 0 <= c - corner <= span: inside the
+            // brick's owned range (own-axis high face only at the grid's end)
+            // and inside the grid (c >= 0 follows from corner >= 0).  In
+            // unsigned arithmetic that is one compare, and a saturated base
+            // index wraps to a value above every span.
+            const uint32_t spx = (uint32_t)min(a == 0 ? ownx_hi : kGBX - 1, (int32_t)dim[0] - 1 - x0);
+            const uint32_t spy = (uint32_t)min(a == 1 ? owny_hi : kGBY - 1, (int32_t)dim[1] - 1 - y0);
+            const uint32_t spz = (uint32_t)min(a == 2 ? ownz_hi : kGBZ - 1, (int32_t)dim[2] - 1 - z0);
             // The taps of p2g_taps (sh/scatter.wgsl:78-118) as straight-line
-            // code: per dimension, whether each of the two face indices is in
-            // the grid and owned by this brick; a tap is added when all three
-            // hold and its weight is positive.  Same expressions, same sums.
+            // code: per dimension, whether each of the two face indices is
+            // owned by this brick; a tap is added when all three are.  Same
+            // expressions, same sums.
             const int32_t bx = sat_i32(floorf(g[0])), by = sat_i32(floorf(g[1])), bz = sat_i32(floorf(g[2]));
             const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
-            bool ox[2], oy[2], oz[2];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                const int32_t ii = bx + d, jj = by + d, kk = bz + d;
-                ox[d] = ii >= 0 && ii < (int32_t)dim[0] && ii - x0 >= 0 && ii - x0 <= hx;
-                oy[d] = jj >= 0 && jj < (int32_t)dim[1] && jj - y0 >= 0 && jj - y0 <= hy;
-                oz[d] = kk >= 0 && kk < (int32_t)dim[2] && kk - z0 >= 0 && kk - z0 <= hz;
-            }
+            const uint32_t ux = (uint32_t)bx - (uint32_t)x0, uy = (uint32_t)by - (uint32_t)y0,
+                           uz = (uint32_t)bz - (uint32_t)z0;
+            const bool ox[2] = {ux <= spx, ux + 1u <= spx}, oy[2] = {uy <= spy, uy + 1u <= spy},
+                       oz[2] = {uz <= spz, uz + 1u <= spz};
             if (!((ox[0] || ox[1]) && (oy[0] || oy[1]) && (oz[0] || oz[1]))) continue;
             const float vcomp = pv[a];
             const float wx[2] = {1.0f - tx, tx}, wy[2] = {1.0f - ty, ty}, wz[2] = {1.0f - tz, tz};
             float wxy[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) wxy[q] = wx[q & 1] * wy[q >> 1];
-            const int32_t li0 = base + (bx - x0) + lx_dim * ((by - y0) + ly_dim * (bz - z0));
-            // Conversions to fixed point.  w is in (0, 1] and P.scale is the
-            // fixed 2^16, so the weight term never saturates; the velocity
-            // term cannot either when |v| * scale < 2e9 (|fl(w v)| <= |v|),
-            // which is false for NaN.  Where a conversion cannot saturate the
-            // plain one equals sat_i32, so only lanes with extreme or NaN
-            // velocities take the saturating loop.
-            auto add_taps = [&](auto plain_v) {
+            const uint32_t li0 = base + ux + (uint32_t)lx_dim * (uy + (uint32_t)ly_dim * uz);
+            // Fixed point, rint then i32 (sh/scatter.wgsl:78-118).  The
+            // reference skips taps with w <= 0; here w is never negative on an
+            // owned tap, and w == 0 (or NaN, from a NaN position) converts to
+            // 0 in both terms, so adding it changes no sum.  P.scale is a
+            // power of two of at most 2^22 (2^16; 2^22 where the adjoint
+            // replays a step).  The short form (moderate position,
+            // |v| < 1e30): v * scale is exact, so rint(w * (v * scale))
+            // equals rint((w * v) * scale) (both are 0 where w * v is
+            // subnormal); w * scale is exact and at most 2^22, so one fused
+            // add of 2^23 rounds it to the nearest even integer, which then
+            // sits in the low bits.  Other lanes take the reference's
+            // expressions with saturating conversions.
+            const float vs = vcomp * P.scale;
+            auto add_taps = [&](auto short_form) {
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
                     const float w = wxy[t & 3] * wz[dk];  // wx * wy * wz
-                    if (ox[di] && oy[dj] && oz[dk] && w > 0.0f) {
-                        const int32_t li = li0 + di + lx_dim * (dj + ly_dim * dk);
-                        const float tv = rintf(w * vcomp * P.scale);
-                        atomicAdd(&s_num[li], decltype(plain_v)::value ? (int32_t)tv : sat_i32(tv));
-                        atomicAdd(&s_den[li], (int32_t)rintf(w * P.scale));
+                    if (ox[di] && oy[dj] && oz[dk]) {
+                        const uint32_t li = li0 + (uint32_t)(di + lx_dim * (dj + ly_dim * dk));
+                        int32_t tn, td;
+                        if constexpr (decltype(short_form)::value) {
+                            tn = sat_i32(rintf(w * vs));
+                            td = __float_as_int(__builtin_fmaf(w, P.scale, 8388608.0f)) - 0x4B000000;
+                        } else {
+                            tn = sat_i32(rintf(w * vcomp * P.scale));
+                            td = sat_i32(rintf(w * P.scale));
+                        }
+                        atomicAdd(&s_num[li], tn);
+                        atomicAdd(&s_den[li], td);
                     }
                 }
             };
-            if (fabsf(vcomp) * P.scale < 2.0e9f)
+            if (moderate && fabsf(vcomp) < 1.0e30f)
                 add_taps(std::true_type{});
             else
                 add_taps(std::false_type{});
@@ -247,7 +285,12 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
             const int32_t i = x0 + lx, j = y0 + ly, k = z0 + lz;
             if (i >= (int32_t)dim[0] || j >= (int32_t)dim[1] || k >= (int32_t)dim[2]) continue;
             float vv, ss;
-            face_epilogue(P, a, ct, (uint32_t)i, (uint32_t)j, (uint32_t)k, s_num[base + t], s_den[base + t], vv, ss);
+            face_epilogue_by(P, a, (uint32_t)i, (uint32_t)j, (uint32_t)k, s_num[base + t], s_den[base + t], vv, ss,
+                             [&](uint32_t ci, uint32_t cj, uint32_t ck) {
+                                 return s_ct[(ci - (uint32_t)(x0 - 1)) +
+                                             (uint32_t)kCX * ((cj - (uint32_t)(y0 - 1)) +
+                                                              (uint32_t)kCY * (ck - (uint32_t)(z0 - 1)))];
+                             });
             const uint32_t f = (uint32_t)i + dim[0] * ((uint32_t)j + dim[1] * (uint32_t)k);
             vel[f] = vv;
             sav[f] = ss;

@@ -486,7 +486,7 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
             FS_HIP(launch_face_epilogue_rows(P, t->num, t->den, t->ct, t->vel, t->saved, s));
         } else {
             FS_HIP(launch_p2g_gather(P, from, from, t->bf16, t->occ, starts, t->ct, t->vel, t->saved,
-                                     t->counters + 1, t->mis_list, s, t->gather_brick, order));
+                                     t->counters + 1, t->mis_list, s, t->gather_brick, order, t->brick_ws));
             // Fallback when more particles' P2G cells differ from their sort
             // cells than the gather walks (any particle order will do).
             FS_HIP(launch_clear_faces_guarded(P, t->num, t->den, t->counters + 1, kMisfitCap, s));
@@ -789,6 +789,13 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         FS_HIP(t->alloc(t->partials, (C + 255) / 256));
         FS_HIP(t->alloc(t->counters, 4));
         FS_HIP(t->alloc(t->mis_list, kMisfitCap));
+        // FLUIDSTEP_GATHER_ORDER=0 runs the gather's bricks in brick order.
+        const char* gorder = std::getenv("FLUIDSTEP_GATHER_ORDER");
+        if (!(gorder && std::atoi(gorder) == 0)) {
+            const size_t nb = 2 * gather_brick_count_max(t->P);
+            FS_HIP(t->alloc(t->brick_ws, nb));
+            FS_HIP(hipMemset(t->brick_ws, 0, nb * sizeof(uint32_t)));
+        }
         const size_t ntiles = C / 64 + 1;
         int dev = 0, cus = 0;
         FS_HIP(hipGetDevice(&dev));

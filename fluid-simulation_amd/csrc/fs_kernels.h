@@ -100,7 +100,10 @@ hipError_t launch_gradient_enforce(const StepParams& P, const float* p, const ui
 // src is the pre-sort side, which the listed misfit indices refer to; the
 // gather does nothing when *mis_count exceeds kMisfitCap (fs_gather.h).  With
 // perm (the stable cell order of `sorted`, whose particles were not moved),
-// sorted slot d is particle perm[d].
+// sorted slot d is particle perm[d].  With brick_ws (gather_brick_count_max
+// * 2 uint32 of workspace, zeroed once) a grid of more bricks than are
+// resident at once runs its bricks in order of decreasing halo particle
+// count, as the previous call counted them.
 // The face epilogue (normalize, save, forces, enforce) of every face from
 // num/den, one row of faces per workgroup.
 hipError_t launch_face_epilogue_rows(const StepParams& P, const int32_t* const num[3], const int32_t* const den[3],
@@ -115,7 +118,11 @@ hipError_t launch_p2g_gather(const StepParams& P, const ParticleSoA& sorted, con
                              const uint32_t* occ, const uint32_t* cell_offset, const uint8_t* ct,
                              float* const vel[3], float* const saved[3], const uint32_t* mis_count,
                              const uint32_t* mis_list, hipStream_t s, int brick_variant = 0,
-                             const uint32_t* perm = nullptr);
+                             const uint32_t* perm = nullptr, uint32_t* brick_ws = nullptr);
+// Workspace of launch_p2g_gather's brick order: 2 * this many uint32 (the
+// largest brick count over the brick shapes); without it the bricks run in
+// brick order.
+size_t gather_brick_count_max(const StepParams& P);
 
 // ---- k_cg.hip: pressure solve ----
 // Reference-structured CG passes (sh/cg_*.wgsl), f32 scalars in `scal`.

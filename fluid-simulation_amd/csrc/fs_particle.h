@@ -82,27 +82,40 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
     }
 }
 
-// G2P + FLIP/PIC blend + CFL clamp + wall friction + advection + wall clamp
-// for particle p (sh/g2p.wgsl:131-198), in place.
-// The particle is read from slot q of rd and written to slot p of sp.
+// What G2P reads for one particle: its position and velocity and the
+// sampled (final, saved) grid velocity per axis.
+struct G2PIn {
+    float pos[3], vel[3], pic[3], sav[3];
+};
+
+// The loads and samples of G2P for the particle in slot q of rd.
 template <int BF16>
-__device__ __forceinline__ void g2p_particle_from(const StepParams& P, const ParticleSoA& rd, uint32_t q,
-                                                  const ParticleSoA& sp, uint32_t p, const float* __restrict__ u,
-                                                  const float* __restrict__ v, const float* __restrict__ w,
-                                                  const float* __restrict__ us, const float* __restrict__ vs,
-                                                  const float* __restrict__ ws) {
+__device__ __forceinline__ void g2p_load(const StepParams& P, const ParticleSoA& rd, uint32_t q,
+                                         const float* __restrict__ u, const float* __restrict__ v,
+                                         const float* __restrict__ w, const float* __restrict__ us,
+                                         const float* __restrict__ vs, const float* __restrict__ ws, G2PIn& in) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        in.pos[a] = rd.pos[a][q];
+        in.vel[a] = load_vel<BF16>(rd.vel[a], q);
+    }
+    sample_axis(P, 0, in.pos, u, us, in.pic[0], in.sav[0]);
+    sample_axis(P, 1, in.pos, v, vs, in.pic[1], in.sav[1]);
+    sample_axis(P, 2, in.pos, w, ws, in.pic[2], in.sav[2]);
+}
+
+// FLIP/PIC blend + CFL clamp + wall friction + advection + wall clamp
+// (sh/g2p.wgsl:131-198) of a loaded particle, written to slot p of sp.
+template <int BF16>
+__device__ __forceinline__ void g2p_finish(const StepParams& P, const G2PIn& in, const ParticleSoA& sp,
+                                           uint32_t p) {
     const float h = P.h, dt = P.dt, alpha = P.flip;
-    const float Pp[3] = {rd.pos[0][q], rd.pos[1][q], rd.pos[2][q]};
-    const float pv[3] = {load_vel<BF16>(rd.vel[0], q), load_vel<BF16>(rd.vel[1], q), load_vel<BF16>(rd.vel[2], q)};
-    float pic[3], sav[3];
-    sample_axis(P, 0, Pp, u, us, pic[0], sav[0]);
-    sample_axis(P, 1, Pp, v, vs, pic[1], sav[1]);
-    sample_axis(P, 2, Pp, w, ws, pic[2], sav[2]);
+    const float* Pp = in.pos;
     float nv[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float flip = pv[a] + (pic[a] - sav[a]);
-        nv[a] = pic[a] * (1.0f - alpha) + flip * alpha;  // WGSL mix(pic, flip, alpha)
+        const float flip = in.vel[a] + (in.pic[a] - in.sav[a]);
+        nv[a] = in.pic[a] * (1.0f - alpha) + flip * alpha;  // WGSL mix(pic, flip, alpha)
     }
     const float cfl = fmaxf(P.cfl, 1.0f);
     const float maxs = cfl * h / dt;
@@ -146,6 +159,18 @@ __device__ __forceinline__ void g2p_particle_from(const StepParams& P, const Par
         sp.pos[a][p] = np[a];
         store_vel<BF16>(sp.vel[a], p, nv[a]);
     }
+}
+
+// G2P of one particle, read from slot q of rd and written to slot p of sp.
+template <int BF16>
+__device__ __forceinline__ void g2p_particle_from(const StepParams& P, const ParticleSoA& rd, uint32_t q,
+                                                  const ParticleSoA& sp, uint32_t p, const float* __restrict__ u,
+                                                  const float* __restrict__ v, const float* __restrict__ w,
+                                                  const float* __restrict__ us, const float* __restrict__ vs,
+                                                  const float* __restrict__ ws) {
+    G2PIn in;
+    g2p_load<BF16>(P, rd, q, u, v, w, us, vs, ws, in);
+    g2p_finish<BF16>(P, in, sp, p);
 }
 
 template <int BF16>

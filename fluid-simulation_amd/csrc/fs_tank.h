@@ -46,6 +46,7 @@ struct fs_tank {
     float* partials = nullptr;  // reference CG partials
     uint32_t* counters = nullptr;  // [0] liquid cells, [1] misfits, [2] CG iterations (reference engine)
     uint32_t* mis_list = nullptr;  // indices (pre-sort side) of the first kMisfitCap misfit particles
+    uint32_t* brick_ws = nullptr;  // gather brick weights and dispatch order (null: brick order)
     CgWork cg{};
     // Persistent-solve vectors of the forward step (b, p, r|r1, cg_w, cg_s, d0).
     fs::CgVectors cg_vectors() const {

@@ -381,12 +381,14 @@ def test_saturating_p2g_velocities(engine):
     """A few particles fast enough that w * v * 2^16 leaves the i32 range:
     the P2G conversion saturates as WGSL's i32() does, on the gather's
     saturating path (the plain conversion is taken only where it cannot
-    saturate).  The integer sums, hence `saved`, stay bit-exact."""
+    saturate).  One particle at 3e38 overflows w * v * 2^16 to infinity.
+    The integer sums, hence `saved`, stay bit-exact."""
     sc = scenes.small_tank(n=16, seed=11)
     vel = np.array(sc["vel"], np.float32, copy=True)
     fast = np.arange(0, len(vel), max(1, len(vel) // 7))[:7]
     vel[fast] = np.array([[2.0e5, -3.0e5, 1.0e9]] * len(fast), np.float32)
     vel[fast[1::2], 0] = -4.0e4
+    vel[fast[2], 1] = -3.0e38
     say("saturating P2G", engine, "fast particles", len(fast))
     ref, t = run_pair(sc, engine, vel=vel)
     assert np.array_equal(t.cell_type(), ref.cell_type)
