@@ -167,7 +167,15 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
             float wxy[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) wxy[q] = wx[q & 1] * wy[q >> 1];
+            // LDS slots of the taps: constant offsets from s_num + li0 (and
+            // s_den + li0).  The empty asm keeps the byte offset computed once
+            // per axis; otherwise the compiler sinks the shift into each tap's
+            // branch.
             const uint32_t li0 = base + ux + (uint32_t)lx_dim * (uy + (uint32_t)ly_dim * uz);
+            uint32_t lb = li0 * 4u;
+            asm volatile("" : "+v"(lb));
+            int32_t* pn = (int32_t*)((char*)s_num + lb);
+            int32_t* pd = (int32_t*)((char*)s_den + lb);
             // Fixed point, rint then i32 (sh/scatter.wgsl:78-118).  The
             // reference skips taps with w <= 0; here w is never negative on an
             // owned tap, and w == 0 (or NaN, from a NaN position) converts to
@@ -187,7 +195,7 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
                     const int di = t & 1, dj = (t >> 1) & 1, dk = t >> 2;
                     const float w = wxy[t & 3] * wz[dk];  // wx * wy * wz
                     if (ox[di] && oy[dj] && oz[dk]) {
-                        const uint32_t li = li0 + (uint32_t)(di + lx_dim * (dj + ly_dim * dk));
+                        const int32_t lo = di + lx_dim * (dj + ly_dim * dk);
                         int32_t tn, td;
                         if constexpr (decltype(short_form)::value) {
                             tn = sat_i32(rintf(w * vs));
@@ -196,8 +204,8 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
                             tn = sat_i32(rintf(w * vcomp * P.scale));
                             td = sat_i32(rintf(w * P.scale));
                         }
-                        atomicAdd(&s_num[li], tn);
-                        atomicAdd(&s_den[li], td);
+                        atomicAdd(pn + lo, tn);
+                        atomicAdd(pd + lo, td);
                     }
                 }
             };
