@@ -18,17 +18,6 @@ __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Element i of an array as a 32-bit byte offset from a uniform base, so the
-// access is one VGPR offset on an SGPR base instead of a 64-bit address per
-// lane (arrays are below 4 GB: grids are limited to 2^30 faces).
-template <typename T>
-__device__ __forceinline__ T ld_at(const T* base, uint32_t i) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ void st_at(T* base, uint32_t i, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
-}
 
 template <int THREADS>
 __device__ __forceinline__ double block_sum(double v, double* s_w) {

@@ -37,6 +37,19 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) {
     return fminf(fmaxf(x, lo), hi);  // WGSL clamp; fmin/fmax drop a NaN operand
 }
 
+// Element i of an array as a 32-bit byte offset from a uniform base, so the
+// access is one VGPR offset on an SGPR base instead of a 64-bit address per
+// lane (arrays are below 4 GB: grids are limited to 2^30 faces, tanks to
+// 2^30 particles).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T* base, uint32_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
 // WGSL i32(f32): truncating, saturating, NaN -> 0.  v_cvt_i32_f32 does
 // exactly that (out-of-range values clamp to INT_MIN/INT_MAX, NaN gives 0),
 // so the conversion is that one instruction; the C cast is undefined out of

@@ -134,13 +134,13 @@ __device__ __forceinline__ void p2g_taps(const float g[3], const uint32_t dim[3]
 
 template <int BF16>
 __device__ __forceinline__ float load_vel(const void* p, uint32_t i) {
-    if (BF16) return bf16_to_f32(((const uint16_t*)p)[i]);
-    return ((const float*)p)[i];
+    if (BF16) return bf16_to_f32(ld_at((const uint16_t*)p, i));
+    return ld_at((const float*)p, i);
 }
 template <int BF16>
 __device__ __forceinline__ void store_vel(void* p, uint32_t i, float v) {
-    if (BF16) ((uint16_t*)p)[i] = f32_to_bf16(v);
-    else ((float*)p)[i] = v;
+    if (BF16) st_at((uint16_t*)p, i, f32_to_bf16(v));
+    else st_at((float*)p, i, v);
 }
 
 // Map block b of a G-block launch so that the blocks one XCD runs (b % 8)

@@ -241,16 +241,16 @@ __device__ __forceinline__ void p2g_gather_brick(const StepParams& P, const Part
         if (perm) {
 #pragma unroll
             for (int k = 0; k < kGBatch; ++k)
-                if (live[k]) pk[k] = perm[pk[k]];
+                if (live[k]) pk[k] = ld_at(perm, pk[k]);
         }
         float px[kGBatch], py[kGBatch], pz[kGBatch], vx[kGBatch], vy[kGBatch], vz[kGBatch];
 #pragma unroll
         for (int k = 0; k < kGBatch; ++k) {
             if (!live[k]) continue;
             const uint32_t p = pk[k];
-            px[k] = sp.pos[0][p];
-            py[k] = sp.pos[1][p];
-            pz[k] = sp.pos[2][p];
+            px[k] = ld_at(sp.pos[0], p);
+            py[k] = ld_at(sp.pos[1], p);
+            pz[k] = ld_at(sp.pos[2], p);
             vx[k] = load_vel<BF16>(sp.vel[0], p);
             vy[k] = load_vel<BF16>(sp.vel[1], p);
             vz[k] = load_vel<BF16>(sp.vel[2], p);

@@ -53,7 +53,10 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
         ok[2 * rw + 1] = okx[1] && oky[dj] && okz[dk];
         const bool any = ok[2 * rw] || ok[2 * rw + 1];
         // A mask rather than a select, so the offset stays straight-line code.
-        const uint32_t off = ((base + sy * (uint32_t)dj + sz * (uint32_t)dk) * 4u) & (0u - (uint32_t)any);
+        uint32_t off = ((base + sy * (uint32_t)dj + sz * (uint32_t)dk) * 4u) & (0u - (uint32_t)any);
+        // Opaque, so that each load takes it as a 32-bit offset on the array's
+        // scalar base (the compiler otherwise forms 64-bit addresses here).
+        asm volatile("" : "+v"(off));
         const f32x2_a4 a = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(vel) + off);
         const f32x2_a4 b = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(sav) + off);
         fv[2 * rw] = a.x;
@@ -96,7 +99,7 @@ __device__ __forceinline__ void g2p_load(const StepParams& P, const ParticleSoA&
                                          const float* __restrict__ vs, const float* __restrict__ ws, G2PIn& in) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        in.pos[a] = rd.pos[a][q];
+        in.pos[a] = ld_at(rd.pos[a], q);
         in.vel[a] = load_vel<BF16>(rd.vel[a], q);
     }
     sample_axis(P, 0, in.pos, u, us, in.pic[0], in.sav[0]);
@@ -156,7 +159,7 @@ __device__ __forceinline__ void g2p_finish(const StepParams& P, const G2PIn& in,
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        sp.pos[a][p] = np[a];
+        st_at(sp.pos[a], p, np[a]);
         store_vel<BF16>(sp.vel[a], p, nv[a]);
     }
 }

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void k_g2p(StepParams P, ParticleSoA sp, Parti
     // lines a region of particles reads are fetched into one XCD's L2.
     const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= P.N) return;
-    g2p_particle_from<BF16>(P, rd, perm ? perm[p] : p, sp, p, u, v, w, us, vs, ws);
+    g2p_particle_from<BF16>(P, rd, perm ? ld_at(perm, p) : p, sp, p, u, v, w, us, vs, ws);
 }
 
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16, const float* const vel[3],
