@@ -471,6 +471,33 @@ def test_unsorted_substeps_gather_equals_atomic(monkeypatch, n, seed, sort, peri
     a.close(), g.close()
 
 
+@pytest.mark.parametrize("period", [1, 4])
+def test_gather_brick_order_gives_identical_bits(monkeypatch, period):
+    """On grids of more bricks than are resident at once (128^3: 4096 bricks)
+    the gather runs its bricks heavy-first, in an order formed from the
+    previous gather's per-brick particle counts (and from zeroed counts on the
+    first substep).  The order only decides which workgroup runs a brick
+    when; each brick writes its own faces from integer sums, so faces,
+    pressure and particles equal those of brick order bit for bit, on sorted
+    substeps and (period 4) on the unsorted ones, whose gather walks a cell
+    order of the particles."""
+    sc = scenes.high_density_128()
+    outs = []
+    for order in ("1", "0"):
+        monkeypatch.setenv("FLUIDSTEP_GATHER_ORDER", order)
+        say("gather brick order", order, "period", period)
+        t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED, particle_sort_period=period), sc["pos"], sc["vel"])
+        arrs = []
+        for _ in range(4):
+            t.step(1)
+            arrs += [*t.faces(fsm.FACE_SAVED), *t.faces(fsm.FACE_VELOCITY), t.pressure(), *t.particles()]
+        assert t.stats()["p2g_gather"] == 1
+        outs.append(arrs)
+        t.close()
+    for i, (a, b) in enumerate(zip(*outs)):
+        assert bits_equal(a, b), i
+
+
 @pytest.mark.parametrize("scene_name,sort,period", [("dam_break_32", 1, 1), ("high_density_128", 1, 1),
                                                     ("small_tank_24", 1, 4)])
 def test_p2g_forms_give_identical_bits(monkeypatch, scene_name, sort, period):
