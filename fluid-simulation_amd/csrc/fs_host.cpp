@@ -552,9 +552,22 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
         }
         return FS_OK;
     }
+    // The gather's cell order (the sort's permutation and bin starts, or the
+    // unsorted substep's gperm and goff) also serves the row-chunk G2P.  By
+    // default it runs for 2^21 particles or more when the row chunks hold 128
+    // or more on average: with fewer particles, or sparser chunks, its
+    // workgroups are too few or their lanes mostly idle, and one particle per
+    // thread measured faster.
+    const bool rows = t->g2p_rows == 1 || (t->g2p_rows < 0 && P.N >= (1u << 21) &&
+                                           (uint64_t)P.N >= 128u * g2p_row_chunks(P));
     if (sort_now) {
-        FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s, &pre, t->perm));
+        if (rows)
+            FS_HIP(launch_g2p_rows(P, live, pre, t->perm, t->cell_offset, t->occ, 0, t->bf16, t->vel, t->saved, s));
+        else
+            FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s, &pre, t->perm));
         flip_live_side(t);
+    } else if (rows && t->gather_last) {
+        FS_HIP(launch_g2p_rows(P, live, live, t->gperm, t->goff, t->occ, 1, t->bf16, t->vel, t->saved, s));
     } else {
         FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));
     }
@@ -818,6 +831,7 @@ FS_API int fs_create(const fs_params* prm, const float* pos_xyz, const float* ve
         if (const char* env = std::getenv("FLUIDSTEP_CG_MODE")) t->cg_mode = std::strcmp(env, "persistent") == 0 ? 0 : 1;
         if (const char* env = std::getenv("FLUIDSTEP_CG_GRAPH")) t->cg_use_graph = std::atoi(env) != 0;
         if (const char* env = std::getenv("FLUIDSTEP_P2G_FORM")) t->p2g_particle = std::strcmp(env, "particle") == 0;
+        if (const char* env = std::getenv("FLUIDSTEP_G2P_ROWS")) t->g2p_rows = std::atoi(env) != 0 ? 1 : 0;
         if (const char* env = std::getenv("FLUIDSTEP_GATHER_BRICK")) {
             const int v = std::atoi(env);
             if (v >= 0 && v <= 2) t->gather_brick = v;

@@ -223,6 +223,13 @@ hipError_t launch_cg_multi(const StepParams& P, const uint8_t* ct, const float* 
 hipError_t launch_g2p(const StepParams& P, const ParticleSoA& live, int bf16,
                       const float* const vel[3], const float* const saved[3], hipStream_t s,
                       const ParticleSoA* from = nullptr, const uint32_t* perm = nullptr);
+// G2P by row chunks of a cell order: the particles of sorted slots d (order[d]
+// on rd) are updated and written to slot d of wr, or back to order[d] when
+// in_place.  starts/occ: the order's bin starts and the cells' counts.
+uint64_t g2p_row_chunks(const StepParams& P);  // workgroups (row chunks) of launch_g2p_rows
+hipError_t launch_g2p_rows(const StepParams& P, const ParticleSoA& wr, const ParticleSoA& rd, const uint32_t* order,
+                           const uint32_t* starts, const uint32_t* occ, int in_place, int bf16,
+                           const float* const vel[3], const float* const saved[3], hipStream_t s);
 // The reference's 32-byte particle record {pos.xyz, 0, vel.xyz, 0} of the
 // side `sp` into dst (8n floats), and u8 -> u32 widening (k_view.hip).
 hipError_t launch_pack_particles_vec4(const ParticleSoA& sp, uint32_t n, bool bf16, float* dst, hipStream_t s);

@@ -10,61 +10,52 @@ namespace fs {
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
 // sample_u/v/w of sh/g2p.wgsl:56-129: wall-aware trilinear gather of
-// (final, saved) with renormalized weights.
-__device__ __forceinline__ void sample_axis(const StepParams& P, int a, const float Pp[3],
-                                            const float* __restrict__ vel, const float* __restrict__ sav,
-                                            float& ofin, float& osav) {
-    uint32_t dim[3];
-    face_dims(P, a, dim);
+// (final, saved) with renormalized weights, in three parts: the taps
+// (axis_taps), the eight (final, saved) face values (loaded by the caller,
+// from global memory or from a staged copy), and the blend (axis_blend).
+//
+// Tap (di, dj, dk) is used when its face is in the grid and does not touch a
+// static solid (*_face_touches_static_solid, sh/g2p.wgsl:35-54).  Both tests
+// are per dimension: 0 <= i < dim and !(i <= l || i >= n-1) reduce to
+// l < i < n-1.  All loads are issued first (a skipped row is loaded from an
+// in-bounds place), then the taps are accumulated in the loop order of the
+// reference; a skipped tap leaves the sums unchanged (a select, not an added
+// zero), so the result equals the branchy form bit for bit.
+struct AxisTaps {
+    int32_t bx, by, bz;  // base face of the 2x2x2 stencil
+    float tx, ty, tz;
+    bool ok[8];          // tap t = (t & 1, (t >> 1) & 1, t >> 2) used
+};
+
+__device__ __forceinline__ void axis_taps(const StepParams& P, int a, const float Pp[3], AxisTaps& T) {
     float g[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) g[c] = (Pp[c] - P.origin[c]) * P.inv_h + stagger(a, c);
-    const int32_t bx = sat_i32(floorf(g[0])), by = sat_i32(floorf(g[1])), bz = sat_i32(floorf(g[2]));
-    const float tx = g[0] - (float)bx, ty = g[1] - (float)by, tz = g[2] - (float)bz;
+    T.bx = sat_i32(floorf(g[0]));
+    T.by = sat_i32(floorf(g[1]));
+    T.bz = sat_i32(floorf(g[2]));
+    T.tx = g[0] - (float)T.bx;
+    T.ty = g[1] - (float)T.by;
+    T.tz = g[2] - (float)T.bz;
     const int32_t nx = (int32_t)P.nx, ny = (int32_t)P.ny, nz = (int32_t)P.nz;
     const int32_t lx = a == 0 ? 1 : 0, ly = a == 1 ? 1 : 0, lz = a == 2 ? 1 : 0;
-    // Tap (di, dj, dk) is used when its face is in the grid and does not
-    // touch a static solid (*_face_touches_static_solid, sh/g2p.wgsl:35-54).
-    // Both tests are per dimension: 0 <= i < dim and !(i <= l || i >= n-1)
-    // reduce to l < i < n-1.  All loads are issued first, at offset 0 for a
-    // skipped row (in bounds), then the taps are accumulated in the loop order
-    // of the reference; a skipped tap leaves the sums unchanged (a select, not
-    // an added zero), so the result equals the branchy form bit for bit.
-    // Offsets are 32-bit bytes from the array base.
     bool okx[2], oky[2], okz[2];
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
-        okx[d] = bx + d > lx && bx + d < nx - 1;
-        oky[d] = by + d > ly && by + d < ny - 1;
-        okz[d] = bz + d > lz && bz + d < nz - 1;
+        okx[d] = T.bx + d > lx && T.bx + d < nx - 1;
+        oky[d] = T.by + d > ly && T.by + d < ny - 1;
+        okz[d] = T.bz + d > lz && T.bz + d < nz - 1;
     }
-    const uint32_t base = (uint32_t)bx + dim[0] * ((uint32_t)by + dim[1] * (uint32_t)bz);
-    const uint32_t sy = dim[0], sz = dim[0] * dim[1];
-    bool ok[8];
-    float fv[8], sw[8];
-    // The two x-taps of a (y, z) row are adjacent faces: one 8-byte load each
-    // from vel and sav.  A row is loaded when either of its taps is used;
-    // then both of its faces are in the grid (a used tap at bx + 1 means
-    // bx >= l >= 0, one at bx means bx + 1 <= n - 1).
 #pragma unroll
     for (int rw = 0; rw < 4; ++rw) {
-        const int dj = rw & 1, dk = rw >> 1;
-        ok[2 * rw] = okx[0] && oky[dj] && okz[dk];
-        ok[2 * rw + 1] = okx[1] && oky[dj] && okz[dk];
-        const bool any = ok[2 * rw] || ok[2 * rw + 1];
-        // A mask rather than a select, so the offset stays straight-line code.
-        uint32_t off = ((base + sy * (uint32_t)dj + sz * (uint32_t)dk) * 4u) & (0u - (uint32_t)any);
-        // Opaque, so that each load takes it as a 32-bit offset on the array's
-        // scalar base (the compiler otherwise forms 64-bit addresses here).
-        asm volatile("" : "+v"(off));
-        const f32x2_a4 a = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(vel) + off);
-        const f32x2_a4 b = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(sav) + off);
-        fv[2 * rw] = a.x;
-        fv[2 * rw + 1] = a.y;
-        sw[2 * rw] = b.x;
-        sw[2 * rw + 1] = b.y;
+        T.ok[2 * rw] = okx[0] && oky[rw & 1] && okz[rw >> 1];
+        T.ok[2 * rw + 1] = okx[1] && oky[rw & 1] && okz[rw >> 1];
     }
-    const float wx[2] = {1.0f - tx, tx}, wy[2] = {1.0f - ty, ty}, wz[2] = {1.0f - tz, tz};
+}
+
+__device__ __forceinline__ void axis_blend(const AxisTaps& T, const float fv[8], const float sw[8], float& ofin,
+                                           float& osav) {
+    const float wx[2] = {1.0f - T.tx, T.tx}, wy[2] = {1.0f - T.ty, T.ty}, wz[2] = {1.0f - T.tz, T.tz};
     float wxy[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) wxy[t] = wx[t & 1] * wy[t >> 1];
@@ -72,9 +63,9 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         const float w = wxy[t & 3] * wz[t >> 2];  // wx * wy * wz
-        fin = ok[t] ? fin + w * fv[t] : fin;
-        sv = ok[t] ? sv + w * sw[t] : sv;
-        wsum = ok[t] ? wsum + w : wsum;
+        fin = T.ok[t] ? fin + w * fv[t] : fin;
+        sv = T.ok[t] ? sv + w * sw[t] : sv;
+        wsum = T.ok[t] ? wsum + w : wsum;
     }
     if (wsum > 0.0f) {
         ofin = fin / wsum;
@@ -83,6 +74,46 @@ __device__ __forceinline__ void sample_axis(const StepParams& P, int a, const fl
         ofin = 0.0f;
         osav = 0.0f;
     }
+}
+
+// The eight face values of the stencil from global memory.  The two x-taps of
+// a (y, z) row are adjacent faces: one 8-byte load each from vel and sav.  A
+// row is loaded when either of its taps is used; then both of its faces are
+// in the grid (a used tap at bx + 1 means bx >= l >= 0, one at bx means
+// bx + 1 <= n - 1).  Offsets are 32-bit bytes from the array base.
+__device__ __forceinline__ void axis_load_global(const StepParams& P, int a, const AxisTaps& T,
+                                                 const float* __restrict__ vel, const float* __restrict__ sav,
+                                                 float fv[8], float sw[8]) {
+    uint32_t dim[3];
+    face_dims(P, a, dim);
+    const uint32_t base = (uint32_t)T.bx + dim[0] * ((uint32_t)T.by + dim[1] * (uint32_t)T.bz);
+    const uint32_t sy = dim[0], sz = dim[0] * dim[1];
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw) {
+        const int dj = rw & 1, dk = rw >> 1;
+        const bool any = T.ok[2 * rw] || T.ok[2 * rw + 1];
+        // A mask rather than a select, so the offset stays straight-line code.
+        uint32_t off = ((base + sy * (uint32_t)dj + sz * (uint32_t)dk) * 4u) & (0u - (uint32_t)any);
+        // Opaque, so that each load takes it as a 32-bit offset on the array's
+        // scalar base (the compiler otherwise forms 64-bit addresses here).
+        asm volatile("" : "+v"(off));
+        const f32x2_a4 va = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(vel) + off);
+        const f32x2_a4 vb = *reinterpret_cast<const f32x2_a4*>(reinterpret_cast<const char*>(sav) + off);
+        fv[2 * rw] = va.x;
+        fv[2 * rw + 1] = va.y;
+        sw[2 * rw] = vb.x;
+        sw[2 * rw + 1] = vb.y;
+    }
+}
+
+__device__ __forceinline__ void sample_axis(const StepParams& P, int a, const float Pp[3],
+                                            const float* __restrict__ vel, const float* __restrict__ sav,
+                                            float& ofin, float& osav) {
+    AxisTaps T;
+    axis_taps(P, a, Pp, T);
+    float fv[8], sw[8];
+    axis_load_global(P, a, T, vel, sav, fv, sw);
+    axis_blend(T, fv, sw, ofin, osav);
 }
 
 // What G2P reads for one particle: its position and velocity and the

@@ -58,6 +58,7 @@ struct fs_tank {
     int cg_use_graph = 1;     // replay the multi-kernel solve from a captured hipGraph
     int gather_brick = 0;     // P2G gather brick: 0 = 8x8x8, 1 = 16x8x8, 2 = 32x4x4 cells
     int p2g_particle = 0;     // 1: the particle-owned brick scatter instead of the face-owning gather
+    int g2p_rows = -1;        // G2P by row chunks with the faces staged in LDS: 1 on, 0 off, -1 for large dense tanks
     hipGraphExec_t cg_exec = nullptr;
 
     uint64_t tick = 0;  // sort tick (advance_sort_tick, src/gpu/fluid.rs:1491-1498)

@@ -250,6 +250,13 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, 
 
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort over 8-bit digits.
+//
+// Per-tile digit counts.  A wave's 64 lanes read consecutive keys, and the
+// keys come nearly sorted (the particles were sorted on an earlier substep and
+// moved little), so neighbouring lanes mostly share a digit: one LDS atomic per
+// run of equal digits (hist_key_runs) instead of one per key, which would
+// serialize on the shared bin.  The counts are integers, so the table is the
+// same either way.
 template <int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                         uint32_t shift, uint32_t ntiles,
@@ -260,8 +267,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
     const size_t base = (size_t)blockIdx.x * (ITEMS * kBlock);
 #pragma unroll 4
     for (int i = 0; i < ITEMS; ++i) {
-        size_t idx = base + threadIdx.x + (size_t)i * kBlock;
-        if (idx < n) atomicAdd(&s_h[(keys[idx] >> shift) & 255u], 1u);
+        const size_t idx = base + threadIdx.x + (size_t)i * kBlock;
+        const bool valid = idx < n;
+        hist_key_runs(valid ? (keys[idx] >> shift) & 255u : 0u, valid, s_h);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
