@@ -261,10 +261,14 @@ template <int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                         uint32_t shift, uint32_t ntiles,
                                                         uint32_t* __restrict__ hist) {
+    // XCD-contiguous tiles, as in k_radix_scatter: neighbouring tiles' table
+    // entries share lines.
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
     __shared__ uint32_t s_h[kRadixBins];
     s_h[threadIdx.x] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * (ITEMS * kBlock);
+    const size_t base = (size_t)tile * (ITEMS * kBlock);
 #pragma unroll 4
     for (int i = 0; i < ITEMS; ++i) {
         const size_t idx = base + threadIdx.x + (size_t)i * kBlock;
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
         hist_key_runs(valid ? (keys[idx] >> shift) & 255u : 0u, valid, s_h);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+    hist[(size_t)threadIdx.x * ntiles + tile] = s_h[threadIdx.x];
 }
 
 // Each wave ranks 64*ITEMS consecutive keys in ITEMS rounds of 64; equal
@@ -298,11 +302,15 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     __shared__ uint32_t s_gbase[kRadixBins];  // global start of each digit's run of this tile
     __shared__ uint32_t s_wtot[kRadixWaves];
     __shared__ uint32_t s_k[kStage ? kTileKeys : 1], s_v[kStage ? kTileKeys : 1];
+    // Each XCD takes a contiguous share of the tiles, so the runs that
+    // neighbouring tiles write side by side are completed in one L2.
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
     for (int i = threadIdx.x; i < kRadixWaves * kRadixBins; i += kBlock) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    const size_t tile_base = (size_t)blockIdx.x * kTileKeys;
+    const size_t tile_base = (size_t)tile * kTileKeys;
     const size_t base = tile_base + (size_t)w * (ITEMS * 64);
     uint32_t kk[ITEMS], vv[ITEMS], rk[ITEMS];
 #pragma unroll
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
             s_cnt[ww][t] = tot;
             tot += c;
         }
-        s_gbase[t] = offsets[(size_t)t * ntiles + blockIdx.x];
+        s_gbase[t] = offsets[(size_t)t * ntiles + tile];
         uint32_t inc = tot;
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t up = __shfl_up(inc, o, 64);
@@ -399,6 +407,7 @@ hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t 
     while (bits < 32 && (1ull << bits) < (uint64_t)key_limit) ++bits;
     const uint32_t passes = (bits + 7) / 8;
     const uint32_t ntiles = (n + tile - 1) / tile;
+    const uint32_t grid8 = (ntiles + 7u) / 8u * 8u;  // whole XCD groups for xcd_remap
     const size_t table = (size_t)ntiles * kRadixBins;
     uint32_t* hist = ws;
     uint32_t* offs = ws + table;
@@ -413,24 +422,24 @@ hipError_t launch_stable_permutation(const uint32_t* keys, uint32_t n, uint32_t 
     for (uint32_t ps = 0; ps < passes; ++ps) {
         const uint32_t shift = 8 * ps;
         if (items == 16)
-            FS_LAUNCH(k_radix_hist<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+            FS_LAUNCH(k_radix_hist<16>, dim3(grid8), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         else if (items == 8)
-            FS_LAUNCH(k_radix_hist<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+            FS_LAUNCH(k_radix_hist<8>, dim3(grid8), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         else
-            FS_LAUNCH(k_radix_hist<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
+            FS_LAUNCH(k_radix_hist<32>, dim3(grid8), dim3(kBlock), 0, s, kin, n, shift, ntiles, hist);
         hipError_t e = launch_exclusive_scan(hist, offs, table, scan_ws, s);
         if (e != hipSuccess) return e;
         uint32_t* kout = kbuf[ps & 1];
         uint32_t* vout = vbuf[ps & 1];
         const int write_keys = ps + 1 < passes;
         if (items == 16)
-            FS_LAUNCH(k_radix_scatter<16>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+            FS_LAUNCH(k_radix_scatter<16>, dim3(grid8), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
                                n, shift, ntiles, write_keys);
         else if (items == 8)
-            FS_LAUNCH(k_radix_scatter<8>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+            FS_LAUNCH(k_radix_scatter<8>, dim3(grid8), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
                                n, shift, ntiles, write_keys);
         else
-            FS_LAUNCH(k_radix_scatter<32>, dim3(ntiles), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
+            FS_LAUNCH(k_radix_scatter<32>, dim3(grid8), dim3(kBlock), 0, s, kin, vin, kout, vout, offs,
                                n, shift, ntiles, write_keys);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
