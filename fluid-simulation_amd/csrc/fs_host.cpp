@@ -554,19 +554,23 @@ int substep_fused(fs_tank* t, bool sort_now, bool stop_before_g2p) {
     }
     // The gather's cell order (the sort's permutation and bin starts, or the
     // unsorted substep's gperm and goff) also serves the row-chunk G2P.  By
-    // default it runs for 2^21 particles or more when the row chunks hold 128
-    // or more on average: with fewer particles, or sparser chunks, its
-    // workgroups are too few or their lanes mostly idle, and one particle per
-    // thread measured faster.
-    const bool rows = t->g2p_rows == 1 || (t->g2p_rows < 0 && P.N >= (1u << 21) &&
-                                           (uint64_t)P.N >= 128u * g2p_row_chunks(P));
+    // default it runs on sort substeps of 2^21 particles or more when the row
+    // chunks hold 128 or more on average: with fewer particles, or sparser
+    // chunks, its workgroups are too few or their lanes mostly idle, and one
+    // particle per thread measured faster.  On a substep without a sort it
+    // would read and write the particles in place through the cell order,
+    // both scattered, where one particle per thread in storage order reads
+    // and writes them contiguously and measured faster.
+    const bool rows_forced = t->g2p_rows == 1;
+    const bool rows = rows_forced || (t->g2p_rows < 0 && P.N >= (1u << 21) &&
+                                      (uint64_t)P.N >= 128u * g2p_row_chunks(P));
     if (sort_now) {
         if (rows)
             FS_HIP(launch_g2p_rows(P, live, pre, t->perm, t->cell_offset, t->occ, 0, t->bf16, t->vel, t->saved, s));
         else
             FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s, &pre, t->perm));
         flip_live_side(t);
-    } else if (rows && t->gather_last) {
+    } else if (rows_forced && t->gather_last) {
         FS_HIP(launch_g2p_rows(P, live, live, t->gperm, t->goff, t->occ, 1, t->bf16, t->vel, t->saved, s));
     } else {
         FS_HIP(launch_g2p(P, live, t->bf16, t->vel, t->saved, s));

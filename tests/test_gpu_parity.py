@@ -507,15 +507,20 @@ def test_g2p_row_chunks_give_identical_bits(monkeypatch, scene_name, period):
     memory: the same face values in the same arithmetic, so faces, pressure
     and particles agree bit for bit over four substeps, on sorted substeps
     (the permutation applied as G2P writes) and (period 4) on unsorted ones
-    (written back in place through the cell order).  The 24^3 tank's h is not
-    a power of two, so some particles' stencils leave their sort cell's rows."""
+    (written back in place through the cell order), with the row chunks
+    forced on every substep, forced off, and as chosen by default (at 128^3:
+    row chunks on the sort substeps only).  The 24^3 tank's h is not a power
+    of two, so some particles' stencils leave their sort cell's rows."""
     if scene_name == "small_tank_24":
         sc = scenes.small_tank(n=24, seed=21, fill=0.5)
     else:
         sc = getattr(scenes, scene_name)()
     outs = []
-    for rows in ("1", "0"):
-        monkeypatch.setenv("FLUIDSTEP_G2P_ROWS", rows)
+    for rows in ("1", "0", None):
+        if rows is None:
+            monkeypatch.delenv("FLUIDSTEP_G2P_ROWS", raising=False)
+        else:
+            monkeypatch.setenv("FLUIDSTEP_G2P_ROWS", rows)
         say("g2p rows", scene_name, rows, "period", period)
         t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED, particle_sort_period=period), sc["pos"], sc["vel"])
         arrs = []
@@ -524,8 +529,8 @@ def test_g2p_row_chunks_give_identical_bits(monkeypatch, scene_name, period):
             arrs += [*t.faces(fsm.FACE_SAVED), *t.faces(fsm.FACE_VELOCITY), t.pressure(), *t.particles()]
         outs.append(arrs)
         t.close()
-    for i, (a, b) in enumerate(zip(*outs)):
-        assert bits_equal(a, b), i
+    for i, (a, b, c) in enumerate(zip(*outs)):
+        assert bits_equal(a, b) and bits_equal(a, c), i
 
 
 @pytest.mark.parametrize("scene_name,sort,period", [("dam_break_32", 1, 1), ("high_density_128", 1, 1),
