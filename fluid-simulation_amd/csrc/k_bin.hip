@@ -123,27 +123,49 @@ hipError_t launch_key_hist(const StepParams& P, const float* const pos[3], uint3
 // [off[k], off[k] + occ[k]) when occ counted the same keys, so `order` is a
 // permutation grouping the particles by cell in key order; the order inside
 // a cell is whatever the atomics give.
-__global__ void k_cell_order(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ off,
-                             uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < n;
-    const uint32_t key = valid ? keys[p] : 0xffffffffu;
+// A thread takes kOrderItems particles kBlock apart (a wave's lanes still hold
+// consecutive particles), so a wave has that many independent load, atomic
+// and store chains in flight.
+namespace {
+constexpr int kOrderItems = 4;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_order(const uint32_t* __restrict__ keys, uint32_t n,
+                                                       const uint32_t* __restrict__ off,
+                                                       uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+    const uint32_t p0 = blockIdx.x * (kBlock * kOrderItems) + threadIdx.x;
     const uint32_t lane = lane_id();
-    const uint32_t prev = __shfl_up(key, 1, 64);
-    const bool head = valid && (lane == 0 || prev != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t act = __ballot(valid);
-    uint32_t base = 0;
-    if (head) {
-        const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-        const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)(64 - __builtin_clzll(act));
-        base = atomicAdd(&cursor[key], end - lane);
+    uint32_t key[kOrderItems], base[kOrderItems], hl[kOrderItems], ok[kOrderItems];
+#pragma unroll
+    for (int i = 0; i < kOrderItems; ++i) {
+        const uint32_t p = p0 + (uint32_t)i * kBlock;
+        key[i] = p < n ? keys[p] : 0xffffffffu;
     }
-    // The head of this lane's run: the highest head lane at or below it.
-    const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-    const uint32_t hl = upto ? 63u - (uint32_t)__builtin_clzll(upto) : 0u;
-    base = __shfl(base, (int)hl, 64);
-    if (valid) order[off[key] + base + (lane - hl)] = p;
+#pragma unroll
+    for (int i = 0; i < kOrderItems; ++i) {
+        const uint32_t p = p0 + (uint32_t)i * kBlock;
+        const bool valid = p < n;
+        const uint32_t prev = __shfl_up(key[i], 1, 64);
+        const bool head = valid && (lane == 0 || prev != key[i]);
+        const uint64_t heads = __ballot(head);
+        const uint64_t act = __ballot(valid);
+        ok[i] = valid ? off[key[i]] : 0u;
+        base[i] = 0;
+        if (head) {
+            const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+            const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)(64 - __builtin_clzll(act));
+            base[i] = atomicAdd(&cursor[key[i]], end - lane);
+        }
+        // The head of this lane's run: the highest head lane at or below it.
+        const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        hl[i] = upto ? 63u - (uint32_t)__builtin_clzll(upto) : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kOrderItems; ++i) {
+        const uint32_t p = p0 + (uint32_t)i * kBlock;
+        const uint32_t b = (uint32_t)__shfl((int)base[i], (int)hl[i], 64);
+        if (p < n) order[ok[i] + b + (lane - hl[i])] = p;
+    }
 }
 
 hipError_t launch_cell_order(const uint32_t* keys, uint32_t n, const uint32_t* cell_offset, uint32_t* cursor,
@@ -151,7 +173,8 @@ hipError_t launch_cell_order(const uint32_t* keys, uint32_t n, const uint32_t* c
     if (n == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(cursor, 0u, ncells, s);
     if (e != hipSuccess) return e;
-    FS_LAUNCH(k_cell_order, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, cell_offset, cursor, order);
+    FS_LAUNCH(k_cell_order, dim3(grid_for(n, kBlock * kOrderItems)), dim3(kBlock), 0, s, keys, n, cell_offset,
+              cursor, order);
     return hipGetLastError();
 }
 

@@ -498,9 +498,12 @@ def test_gather_brick_order_gives_identical_bits(monkeypatch, period):
         assert bits_equal(a, b), i
 
 
-@pytest.mark.parametrize("scene_name,period", [("high_density_128", 1), ("high_density_128", 4),
-                                               ("dam_break_32", 1), ("small_tank_24", 4)])
-def test_g2p_row_chunks_give_identical_bits(monkeypatch, scene_name, period):
+@pytest.mark.parametrize("scene_name,period,velocity", [("high_density_128", 1, fsm.VELOCITY_F32),
+                                                        ("high_density_128", 4, fsm.VELOCITY_F32),
+                                                        ("dam_break_32", 1, fsm.VELOCITY_F32),
+                                                        ("dam_break_32", 4, fsm.VELOCITY_BF16),
+                                                        ("small_tank_24", 4, fsm.VELOCITY_F32)])
+def test_g2p_row_chunks_give_identical_bits(monkeypatch, scene_name, period, velocity):
     """G2P by row chunks (the faces a chunk's particles read staged in LDS, a
     wave whose stencils leave the staged rows reading that axis from global
     memory) against one particle per thread reading every face from global
@@ -509,8 +512,9 @@ def test_g2p_row_chunks_give_identical_bits(monkeypatch, scene_name, period):
     (the permutation applied as G2P writes) and (period 4) on unsorted ones
     (written back in place through the cell order), with the row chunks
     forced on every substep, forced off, and as chosen by default (at 128^3:
-    row chunks on the sort substeps only).  The 24^3 tank's h is not a power
-    of two, so some particles' stencils leave their sort cell's rows."""
+    row chunks on the sort substeps only), with f32 and (32^3) bf16 particle
+    velocities.  The 24^3 tank's h is not a power of two, so some particles'
+    stencils leave their sort cell's rows."""
     if scene_name == "small_tank_24":
         sc = scenes.small_tank(n=24, seed=21, fill=0.5)
     else:
@@ -521,8 +525,9 @@ def test_g2p_row_chunks_give_identical_bits(monkeypatch, scene_name, period):
             monkeypatch.delenv("FLUIDSTEP_G2P_ROWS", raising=False)
         else:
             monkeypatch.setenv("FLUIDSTEP_G2P_ROWS", rows)
-        say("g2p rows", scene_name, rows, "period", period)
-        t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED, particle_sort_period=period), sc["pos"], sc["vel"])
+        say("g2p rows", scene_name, rows, "period", period, "velocity", velocity)
+        t = fsm.FluidStep(gpu_params(sc, fsm.ENGINE_FUSED, velocity=velocity, particle_sort_period=period),
+                          sc["pos"], sc["vel"])
         arrs = []
         for _ in range(4):
             t.step(1)
